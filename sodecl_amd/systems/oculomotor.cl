@@ -1,0 +1,133 @@
+/*
+ * Oculomotor (saccadic eye movement) model as a SODECL system plugin.
+ *
+ * States (6):  y0 = g (eye position), y1 = u (eye velocity), y2 = n
+ *              (neural integrator), y3 = r and y4 = l (right/left burst
+ *              neurons), y5 = m (motor error).
+ * Parameters (6 per orbit):
+ *              p0, p1  burst-curve amplitude and width for negative input,
+ *              p2      burst-neuron time constant,
+ *              p3      cross-inhibition strength between r and l,
+ *              p4, p5  burst-curve amplitude and width for positive input.
+ * SDE variant: two extra parameters p[_numpar_-2], p[_numpar_-1] scale the
+ *              two noise processes entering the velocity equation.
+ *
+ * The model equations are those of examples/oculomotor.cl:1-112 in the
+ * reference (sode_system :83-97, sode_system_stoch :99-112, calc_jacobian
+ * :38-81, burst curve and its derivative :5-36).  Each right-hand side is
+ * written with the same floating-point operations in the same order, so
+ * trajectories agree with the reference to rounding.
+ *
+ * Plugin contract: OpenCL-C style functions over fixed-size arrays whose
+ * extents are the macros _numeq_, _numpar_ and _numnoi_ supplied by the
+ * integrator.  calc_jacobian_band is an optional entry point read by the
+ * implicit solvers: it returns only the three diagonals the tridiagonal
+ * solve uses (lower[i] = J[i+1][i], diag[i] = J[i][i], upper[i] = J[i][i+1]).
+ */
+
+#define OCU_T1 0.15
+#define OCU_T2 0.012
+#define OCU_TN 25.00
+
+/* Burst-neuron firing curve: saturating exponential for x > 0, a scaled
+ * x*exp(x/b) ramp otherwise. */
+double ocu_burst(double x, double a, double b, double at, double bt)
+{
+    if (x > 0)
+    {
+        return at * (1.00 - exp(-x / bt));
+    }
+    return -(a / b) * x * exp(x / b);
+}
+
+/* d(ocu_burst)/dx */
+double ocu_burst_dx(double x, double a, double b, double at, double bt)
+{
+    if (x > 0)
+    {
+        return (at * exp(-x / bt)) / bt;
+    }
+    return -((a * exp(x / b)) / b + (a * x * exp(x / b)) / (b * b));
+}
+
+void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
+{
+    const double rate_sum = (1.00 / OCU_T1) + (1.00 / OCU_T2);
+    const double rate_prod = 1.00 / (OCU_T1 * OCU_T2);
+    const double drive = y[3] - y[4];
+
+    yout[0] = y[1];
+    yout[1] = -(rate_sum * y[1]) - (rate_prod * y[0]) + (rate_prod * y[2]) + rate_sum * drive;
+    yout[2] = -(y[2] / OCU_TN) + drive;
+    yout[3] = (1.00 / p[2]) * (-y[3] - (p[3] * y[3] * y[4] * y[4]) + ocu_burst(y[5], p[0], p[1], p[4], p[5]));
+    yout[4] = (1.00 / p[2]) * (-y[4] - (p[3] * y[4] * y[3] * y[3]) + ocu_burst(-y[5], p[0], p[1], p[4], p[5]));
+    yout[5] = -drive;
+}
+
+void sode_system_stoch(double t, double y[_numeq_], double stoch[_numeq_], double p[_numpar_], double noise[_numnoi_])
+{
+    const double rate_sum = (1.00 / OCU_T1) + (1.00 / OCU_T2);
+    for (int i = 0; i < _numeq_; i++)
+    {
+        stoch[i] = 0;
+    }
+    stoch[1] = rate_sum * (y[3] - y[4]) * p[_numpar_ - _numnoi_ + 0] * noise[0] + p[_numpar_ - _numnoi_ + 1] * noise[1];
+}
+
+void calc_jacobian(double y[_numeq_], double jac[_numeq_][_numeq_], double p[_numpar_])
+{
+    for (int i = 0; i < _numeq_; i++)
+    {
+        for (int j = 0; j < _numeq_; j++)
+        {
+            jac[i][j] = 0;
+        }
+    }
+    jac[0][1] = 1;
+
+    jac[1][0] = -1 / (OCU_T1 * OCU_T2);
+    jac[1][1] = -(1 / OCU_T1) - (1 / OCU_T2);
+    jac[1][2] = 1 / (OCU_T1 * OCU_T2);
+    jac[1][3] = (1 / OCU_T1) + (1 / OCU_T2);
+    jac[1][4] = -(1 / OCU_T1) - (1 / OCU_T2);
+
+    jac[2][2] = -1 / OCU_TN;
+    jac[2][3] = 1;
+    jac[2][4] = -1;
+
+    jac[3][3] = -(p[3] * (y[4] * y[4]) + 1) / p[2];
+    jac[3][4] = -(2 * p[3] * y[4] * y[3]) / p[2];
+    jac[3][5] = ocu_burst_dx(y[5], p[0], p[1], p[4], p[5]) / p[2];
+
+    jac[4][3] = -(2 * p[3] * y[4] * y[3]) / p[2];
+    jac[4][4] = -(p[3] * (y[3] * y[3]) + 1) / p[2];
+    jac[4][5] = ocu_burst_dx(-y[5], p[0], p[1], p[4], p[5]) / p[2];
+
+    jac[5][3] = -1;
+    jac[5][4] = 1;
+}
+
+void calc_jacobian_band(double y[_numeq_], double lower[_numeq_], double diag[_numeq_], double upper[_numeq_], double p[_numpar_])
+{
+    /* diag[i] = J[i][i] */
+    diag[0] = 0;
+    diag[1] = -(1 / OCU_T1) - (1 / OCU_T2);
+    diag[2] = -1 / OCU_TN;
+    diag[3] = -(p[3] * (y[4] * y[4]) + 1) / p[2];
+    diag[4] = -(p[3] * (y[3] * y[3]) + 1) / p[2];
+    diag[5] = 0;
+    /* upper[i] = J[i][i+1] */
+    upper[0] = 1;
+    upper[1] = 1 / (OCU_T1 * OCU_T2);
+    upper[2] = 1;
+    upper[3] = -(2 * p[3] * y[4] * y[3]) / p[2];
+    upper[4] = ocu_burst_dx(-y[5], p[0], p[1], p[4], p[5]) / p[2];
+    upper[5] = 0;
+    /* lower[i] = J[i+1][i] */
+    lower[0] = -1 / (OCU_T1 * OCU_T2);
+    lower[1] = 0;
+    lower[2] = 0;
+    lower[3] = -(2 * p[3] * y[4] * y[3]) / p[2];
+    lower[4] = 1;
+    lower[5] = 0;
+}
