@@ -1,0 +1,229 @@
+"""Benchmark: orbit-steps/s of the MI355X integrators on one GPU.
+
+Default workload: the 6-state oculomotor model, classic RK4, dt=1e-3, 20000
+steps, a snapshot every 100 steps (201 snapshots written to HBM), 65536
+orbits with seeded log-uniform parameters (sodecl_amd/configs.py).  One
+"step" of this benchmark is one complete integration of every orbit
+(20000 RK4 steps with trajectory writeback), inputs resident in HBM.
+
+Printed (one JSON line on stdout): value = orbits * integrator steps per
+run / wall seconds per run; roofline of the integration kernel from HIP
+events on its launch stream (FP64 algorithmic flops, 78.6 TFLOP/s peak);
+the reference's kernels compiled for the host CPU (oracle/_ref) timed on a
+bounded sample on all host cores; and the reference's launch design on the
+same GPU (one launch per ksteps segment with a blocking state read-back).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+METRIC = "orbit-steps/s (fwd, fwd+adjoint) vs plain kernel, same GPU, at stated tol"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="oculomotor_rk4",
+                    choices=["oculomotor_rk4", "kuramoto_euler", "kuramoto_em", "kuramoto_rk4"])
+    ap.add_argument("--schedule", default="auto")
+    ap.add_argument("--group-size", type=int, default=0)
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-orbits", type=int, default=0, help="orbits in the CPU baseline sample")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured with rocprofv3 counters, if committed."""
+    path = os.path.join(HERE, "profiles", f"{name}_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(cfg, orbits_sample):
+    """The reference kernels compiled for the host (oracle/_ref, -O3), run as
+    the reference's host loop does, OpenMP over orbits on every host core."""
+    import numpy as np
+    from oracle.oracle import RefKernels, have_ref
+    system = os.path.splitext(os.path.basename(cfg["system"]))[0]
+    solver = {0: "em", 1: "euler", 2: "rk4", 3: "ie", 4: "im"}[cfg["solver"]]
+    if not have_ref(system, solver):
+        return None
+    ref = RefKernels(system, solver, fast=True)
+    k = min(orbits_sample, cfg["orbits"])
+    nthreads = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp:
+        nthreads = min(nthreads, int(omp))
+    t0 = time.perf_counter()
+    ref.run(cfg["y0"][:k], cfg["params"][:k], cfg["dt"], cfg["tspan"], cfg["ksteps"],
+            counters=None if cfg["counters"] is None else cfg["counters"][:k], nnoi=cfg["nnoi"],
+            nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    from sodecl_amd import sodecl_interface as si
+    steps = si.num_launches(cfg["tspan"], cfg["dt"], cfg["ksteps"]) * cfg["ksteps"]
+    return dict(value=k * steps / dt, unit="orbit-steps/s", cores=nthreads, kind="reference",
+                sample=f"{k} of {cfg['orbits']} orbits, all {steps} steps with trajectory read-back "
+                       f"({dt:.2f} s wall)",
+                seconds=dt)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(orbits shard with no per-step communication, see DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from sodecl_amd import configs, sodecl_interface as si
+
+    log(f"config {args.config}: building inputs")
+    cfg = getattr(configs, args.config)()
+    orbits, neq, npar = cfg["orbits"], cfg["neq"], cfg["npar"]
+    launches = si.num_launches(cfg["tspan"], cfg["dt"], cfg["ksteps"])
+    nsnap = launches + 1
+    steps_per_run = launches * cfg["ksteps"]
+    opts = {"schedule": args.schedule}
+    if args.group_size:
+        opts["group_size"] = args.group_size
+    if args.block_size:
+        opts["block_size"] = args.block_size
+
+    dev = torch.device("cuda:0")
+    log("compiling and uploading")
+    sess = configs.session_for(cfg, **opts)
+    sess.setup(cfg["y0"], cfg["params"], None, cfg["counters"])
+    y_init = torch.tensor(cfg["y0"], dtype=torch.float64, device=dev).contiguous()
+    p = torch.tensor(cfg["params"], dtype=torch.float64, device=dev).contiguous()
+    y = torch.empty_like(y_init)
+    t = torch.zeros(orbits, dtype=torch.float64, device=dev)
+    cnt_init = None if cfg["counters"] is None else torch.tensor(cfg["counters"], dtype=torch.float64, device=dev)
+    cnt = None if cnt_init is None else torch.empty_like(cnt_init)
+    traj = torch.empty((nsnap, neq, orbits), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    ev = []
+
+    def one_step(record):
+        with torch.cuda.stream(stream):
+            y.copy_(y_init)
+            t.zero_()
+            if cnt is not None:
+                cnt.copy_(cnt_init)
+            if record:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            sess.run_device(t.data_ptr(), y.data_ptr(), p.data_ptr(), 0 if cnt is None else cnt.data_ptr(),
+                            traj.data_ptr(), stream.cuda_stream)
+            if record:
+                e1.record(stream)
+                ev.append((e0, e1))
+
+    for i in range(args.warmup):
+        one_step(False)
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        one_step(True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    log(f"timed {args.steps} steps in {wall:.3f} s")
+    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    ms_per_step = wall * 1e3 / args.steps
+    value = orbits * steps_per_run / (wall / args.steps)
+    kms = sum(kernel_ms) / len(kernel_ms)
+    flops = cfg["flops_per_orbit_step"] * orbits * steps_per_run
+    achieved = flops / (kms * 1e-3) / 1e12
+    sched = sess.launch_config()
+    out = traj[-1].cpu().numpy()
+    finite = float(np.isfinite(out).mean())
+
+    plain = None
+    if not args.no_plain:
+        log("plain kernel (reference launch design) run")
+        ps = configs.session_for(cfg, schedule="plain")
+        ps.setup(cfg["y0"], cfg["params"], None, cfg["counters"])
+        t0 = time.perf_counter()
+        pres = ps.run()
+        pwall = time.perf_counter() - t0
+        plain = dict(value=orbits * steps_per_run / pwall, unit="orbit-steps/s", seconds=pwall,
+                     design="lane per orbit, one launch per ksteps segment, blocking state read-back and "
+                            "host transposition before each launch")
+        got = traj.cpu().numpy().reshape(-1)
+        same = bool(np.array_equal(np.nan_to_num(pres, nan=1e300), np.nan_to_num(got, nan=1e300)))
+        plain["bitwise_equal_to_fast_path"] = same if sched[0] == 1 else None
+        del pres, got
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (reference kernels on host cores)")
+        sample = args.cpu_orbits or {"oculomotor_rk4": 8192, "kuramoto_euler": 1024, "kuramoto_em": 64,
+                                     "kuramoto_rk4": 16}[args.config]
+        cpu = cpu_baseline(cfg, sample)
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "orbit-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded numpy PCG64 draws, sodecl_amd/configs.py)",
+        "config": {
+            "workload": cfg["name"],
+            "system": os.path.basename(cfg["system"]),
+            "solver": {0: "euler-maruyama", 1: "euler", 2: "rk4", 3: "implicit-euler", 4: "implicit-midpoint"}[
+                cfg["solver"]],
+            "orbits": orbits, "neq": neq, "dt": cfg["dt"], "steps": steps_per_run, "ksteps": cfg["ksteps"],
+            "snapshots": nsnap,
+            "schedule": {1: "lane", 2: "group"}.get(sched[0], str(sched[0])), "group_size": sched[1],
+            "block_size": sched[2],
+        },
+        "roofline": {
+            "bound": "fp64-valu",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic_from_profiles(cfg["name"]),
+            "kernel_ms": kms,
+            "flops_per_orbit_step": cfg["flops_per_orbit_step"],
+        },
+        "cpu_baseline": cpu,
+        "plain_kernel": plain,
+        "speedup_vs_plain": None if plain is None else value / plain["value"],
+        "speedup_vs_cpu": None if cpu is None else value / cpu["value"],
+        "finite_fraction_final_state": finite,
+        "tolerance": "1e-10 relative vs the CPU oracle (tests/test_gpu_parity.py)",
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,193 @@
+/*
+ * sodecl.h -- C ABI of the MI355X-native SODECL integrator library
+ * (libsodecl_amd.so).
+ *
+ * The boundary mirrors the reference's host manager, class
+ * sodecl::sodeclmgr (include/sodeclmgr.hpp:27-369, src/sodeclmgr.cpp of
+ * avramidis/sodecl), as plain C: one opaque context per manager object,
+ * plain pointers and sizes, no HIP or torch types.  Every function that can
+ * fail returns 1 on success and 0 on failure, as the reference's member
+ * functions do; sodecl_last_error() then holds the message.
+ *
+ * Ownership: the context owns all device memory and the host output.  Input
+ * pointers given to sodecl_set_t0/y0/params are stashed and read at
+ * sodecl_setup(), the same lifetime rule as the reference
+ * (sodeclmgr.cpp:803-829 stash, :735-770 upload).  One context per thread.
+ *
+ * Layouts (the reference's):
+ *   t0[orbits]; y0[orbits][neq]; params[orbits][npar]; counters[orbits];
+ *   output[nsnap][neq][orbits], nsnap = launches + 1, launches = number of
+ *   integers j >= 0 with j < tspan/(dt*ksteps) (sodeclmgr.cpp:951-1015).
+ */
+#ifndef SODECL_AMD_SODECL_H
+#define SODECL_AMD_SODECL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SODECL_AMD_ABI_VERSION 1
+
+/* include/solver_Type.hpp:14-20 */
+enum sodecl_solver {
+    SODECL_STOCHASTIC_EULER = 0,
+    SODECL_EULER = 1,
+    SODECL_RUNGE_KUTTA = 2,
+    SODECL_IMPLICIT_EULER = 3,
+    SODECL_IMPLICIT_MIDPOINT = 4
+};
+
+/* include/output_Type.hpp:14-18 */
+enum sodecl_output { SODECL_OUTPUT_NONE = 0, SODECL_OUTPUT_ARRAY = 1, SODECL_OUTPUT_FILE = 2 };
+
+/* include/device_Type.hpp:14-19 (values of CL_DEVICE_TYPE_*) */
+#define SODECL_DEVICE_TYPE_CPU 0x2ull
+#define SODECL_DEVICE_TYPE_GPU 0x4ull
+#define SODECL_DEVICE_TYPE_ACCELERATOR 0x8ull
+#define SODECL_DEVICE_TYPE_ALL 0xFFFFFFFFull
+
+/* Kernel schedules (no reference counterpart: the reference always runs one
+ * work-item per orbit and one launch per ksteps segment). */
+enum sodecl_schedule {
+    SODECL_SCHEDULE_AUTO = 0,  /* group when the plugin has per-component entry points
+                                  and the system is large enough, else lane; timed
+                                  choice when autotune is on */
+    SODECL_SCHEDULE_LANE = 1,  /* one orbit per lane, state in registers */
+    SODECL_SCHEDULE_GROUP = 2, /* group_size lanes per orbit, stage vectors in LDS */
+    SODECL_SCHEDULE_PLAIN = 3  /* the reference's launch design: one launch per ksteps
+                                  segment with a blocking state read-back before each */
+};
+
+/* Extra knobs (no reference counterpart).  Zero-initialised fields mean the
+ * default, which is the reference's behaviour. */
+typedef struct sodecl_options {
+    int schedule;            /* enum sodecl_schedule */
+    int group_size;          /* lanes per orbit for the group schedule (power of two <= 64); 0 = auto */
+    int block_size;          /* threads per block; 0 = local_group_size if set, else auto */
+    int newton_max_iter;     /* 0 = 1000 (src/kernels/ie.cl:69, im.cl:65) */
+    int segments_per_launch; /* ksteps segments per kernel launch; 0 = as many as the
+                                trajectory buffer budget allows */
+    int use_graph;           /* replay the launch sequence of a run from a captured hipGraph */
+    int autotune;            /* time the candidate schedules once per code object and keep the fastest */
+    int fp_contract;         /* 0 = fused multiply-add contraction allowed (default), 1 = off */
+    int verbose;             /* print "Compute results runtime" like the reference */
+    size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
+} sodecl_options;
+
+typedef struct sodecl_ctx sodecl_ctx;
+
+/* Replaces sodeclmgr::sodeclmgr (sodeclmgr.hpp:122-124, sodeclmgr.cpp:26-83).
+ * kernel_path: directory of solver sources; the solvers are built into this
+ * library, so it is accepted and ignored.  system_path: the user system
+ * source (OpenCL-C style functions).  RNG counters of SDE runs are drawn as
+ * in the reference (random, per orbit) unless sodecl_set_rng_counters() is
+ * called. */
+int sodecl_create(sodecl_ctx** ctx, const char* kernel_path, const char* system_path, int solver, double dt,
+                  double tspan, int ksteps, int neq, int npar, int nnoi, int orbits, int output_type);
+
+/* As sodecl_create, with the system source given as text instead of a path
+ * (no reference counterpart). */
+int sodecl_create_from_source(sodecl_ctx** ctx, const char* system_source, int solver, double dt, double tspan,
+                              int ksteps, int neq, int npar, int nnoi, int orbits, int output_type);
+
+/* Replaces sodeclmgr::~sodeclmgr (sodeclmgr.cpp:88-149). */
+void sodecl_destroy(sodecl_ctx* ctx);
+
+/* Replaces sodeclmgr::choose_device (sodeclmgr.hpp:144, sodeclmgr.cpp:164-212):
+ * platform must be 0; device is the HIP device ordinal; device_type must be
+ * ALL or GPU. */
+int sodecl_choose_device(sodecl_ctx* ctx, uint32_t platform, uint64_t device_type, uint32_t device);
+
+/* Replaces sodeclmgr::set_local_group_size (sodeclmgr.hpp:151). */
+void sodecl_set_local_group_size(sodecl_ctx* ctx, int local_group_size);
+
+/* Replace sodeclmgr::set_t0/set_y0/set_params (sodeclmgr.hpp:320-334): pointers
+ * are stashed and read at sodecl_setup(). */
+void sodecl_set_t0(sodecl_ctx* ctx, const double* t0);
+void sodecl_set_y0(sodecl_ctx* ctx, const double* y0);
+void sodecl_set_params(sodecl_ctx* ctx, const double* params);
+
+/* Per-orbit SDE counters, doubles as the reference's m_rcounter
+ * (sodeclmgr.cpp:48-59), copied immediately.  Values below 2^53 reproduce the
+ * reference's noise stream exactly.  No reference counterpart (the reference
+ * draws them from std::random_device and never exposes them). */
+int sodecl_set_rng_counters(sodecl_ctx* ctx, const double* counters);
+/* Counters as they stand now (after a run: advanced by the calls made). */
+int sodecl_get_rng_counters(sodecl_ctx* ctx, double* counters);
+
+int sodecl_set_options(sodecl_ctx* ctx, const sodecl_options* options);
+void sodecl_default_options(sodecl_options* options);
+
+/* Replaces sodeclmgr::setup_sode_solver (sodeclmgr.hpp:368,
+ * sodeclmgr.cpp:1042-1097): assembles and JIT-compiles the program, allocates
+ * device memory, uploads inputs. */
+int sodecl_setup(sodecl_ctx* ctx);
+
+/* Replaces sodeclmgr::run_sode_solver (sodeclmgr.hpp:361,
+ * sodeclmgr.cpp:921-1035).  Integrates from the current device state (a
+ * second call continues where the first ended, as the reference's device
+ * buffers do) and appends to the output: host array (ARRAY), raw doubles
+ * appended to sodecloutput.bin in the working directory (FILE), or nothing
+ * (NONE). */
+int sodecl_run(sodecl_ctx* ctx);
+
+/* Output of the runs so far: nsnap*neq*orbits doubles per run, [snap][eq][orbit]
+ * (the reference's public m_output, sodeclmgr.hpp:33). */
+size_t sodecl_output_size(const sodecl_ctx* ctx);
+const double* sodecl_output(const sodecl_ctx* ctx);
+/* Let the next sodecl_run() write its ARRAY output into caller memory of
+ * capacity doubles instead of the context's own buffer. */
+int sodecl_set_output_buffer(sodecl_ctx* ctx, double* host, size_t capacity);
+
+/* Number of kernel-segment launches and snapshots of one run. */
+long sodecl_num_launches(double tspan, double dt, int ksteps);
+
+/* Device-resident run (no reference counterpart): all pointers are device
+ * memory of the context's device.  t0/y0/params/counters as above (updated
+ * in place); traj receives nsnap snapshots [snap][neq][orbits] (may be NULL);
+ * stream is a hipStream_t or NULL for the context's stream.  Returns after
+ * the work is enqueued; synchronise the stream before reading results. */
+int sodecl_run_device(sodecl_ctx* ctx, double* t0, double* y0, const double* params, double* counters,
+                      double* traj, void* stream);
+
+/* Wall time of the last sodecl_run segment loop in seconds, and the kernel
+ * time of the last run measured with HIP events on the launch stream. */
+double sodecl_last_runtime(const sodecl_ctx* ctx);
+double sodecl_last_kernel_ms(const sodecl_ctx* ctx);
+
+/* Schedule, group size and block size the last run used. */
+int sodecl_get_launch_config(const sodecl_ctx* ctx, int* schedule, int* group_size, int* block_size);
+
+/* Newton updates per orbit accumulated over the runs (implicit solvers);
+ * counts[orbits]. */
+int sodecl_get_newton_updates(sodecl_ctx* ctx, long long* counts);
+
+/* Assembles and compiles the program for gfx950 without a device (build
+ * check); returns the code-object size in bytes or 0. */
+size_t sodecl_compile_only(sodecl_ctx* ctx, const char* arch);
+/* The program text the JIT compiles (the counterpart of the reference's
+ * Generated_kernel.cl dump, sodeclmgr.cpp:497-500). */
+const char* sodecl_program_source(sodecl_ctx* ctx);
+
+const char* sodecl_last_error(const sodecl_ctx* ctx);
+
+/* Streaming device-to-device copy of bytes (a multiple of 16) on stream (a
+ * hipStream_t or NULL), by a kernel compiled ahead of time for gfx950.  No
+ * reference counterpart: a kernel of known traffic used to calibrate HBM
+ * byte counters when profiling the integrators. */
+int sodecl_copy_device(void* dst, const void* src, size_t bytes, void* stream);
+
+/* Replace sodeclmgr::read_binary_data_from_file / read_data_from_file
+ * (sodeclmgr.hpp:344,354; sodeclmgr.cpp:839-914).  data must hold capacity
+ * doubles; *count receives the number read.  Return 1/0. */
+int sodecl_read_binary_data_from_file(const char* filename, double* data, size_t capacity, size_t* count);
+int sodecl_read_data_from_file(const char* filename, double* data, size_t capacity, size_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,496 @@
+// SODECL MI355X integrators: device code compiled at run time by hipRTC for
+// gfx950, after the macro prelude and the user's system source.
+//
+// Macros provided by the host (sodecl_amd/csrc/source.cpp):
+//   _numeq_ _numpar_ _numnoi_ _listsize_ _numsteps_ _m_dt_ [_epsilon1_]
+//       the reference's macro contract (src/sodeclmgr.cpp:390-430)
+//   SODECL_SOLVER          0 Euler-Maruyama, 1 Euler, 2 RK4, 3 implicit Euler,
+//                          4 implicit midpoint (include/solver_Type.hpp:14-20)
+//   SODECL_BLOCK           threads per block
+//   SODECL_GROUP           lanes per orbit in the group schedule
+//   SODECL_NEWTON_MAXIT    Newton iteration cap (ie.cl:69 uses 1000)
+//   SODECL_HAS_*           optional plugin entry points found in the source
+//
+// Two schedules:
+//   sodecl_lane   one orbit per lane; state, parameters and stage vectors
+//                 live in registers (small systems such as the oculomotor)
+//   sodecl_group  SODECL_GROUP lanes per orbit; lane l owns components
+//                 l, l+G, ...; stage vectors are staged in LDS and every lane
+//                 evaluates its components through the plugin's per-component
+//                 entry point (coupled systems such as Kuramoto)
+// Both run many ksteps segments per launch and write the snapshot that
+// precedes each segment straight into the trajectory buffer, laid out
+// [snapshot][equation][orbit] as the reference's output (sodeclmgr.cpp:979-995).
+
+#define SODECL_INLINE __device__ __attribute__((always_inline)) inline
+
+#if _numeq_ <= 32
+#define SODECL_UNROLL _Pragma("unroll")
+#else
+#define SODECL_UNROLL
+#endif
+
+#define SODECL_NPAR_ALLOC (_numpar_ > 0 ? _numpar_ : 1)
+#define SODECL_NNOI_ALLOC (_numnoi_ > 0 ? _numnoi_ : 1)
+#define SODECL_IS_SDE (SODECL_SOLVER == 0)
+#define SODECL_IS_IMPLICIT (SODECL_SOLVER == 3 || SODECL_SOLVER == 4)
+
+typedef unsigned long long sodecl_u64;
+
+// ---------------------------------------------------------------------------
+// Threefry-2x64, 20 rounds (Salmon, Moraes, Dror, Shaw, "Parallel Random
+// Numbers: As Easy as 1, 2, 3", SC'11): Threefish-style add/rotate/xor rounds
+// with the Skein 2x64 rotation schedule and a key injection after every four
+// rounds.  Identical output to Random123's threefry2x64 (threefry.h:178-293),
+// which the reference's SDE caller uses.
+// ---------------------------------------------------------------------------
+SODECL_INLINE sodecl_u64 sodecl_rotl(sodecl_u64 v, int n) { return (v << n) | (v >> (64 - n)); }
+
+#define SODECL_TF_ROUND(R) \
+    x0 += x1;              \
+    x1 = sodecl_rotl(x1, R); \
+    x1 ^= x0;
+
+SODECL_INLINE void sodecl_threefry2x64_20(sodecl_u64 c0, sodecl_u64 c1, sodecl_u64 k0, sodecl_u64 k1,
+                                          sodecl_u64* o0, sodecl_u64* o1)
+{
+    const sodecl_u64 k2 = 0x1BD11BDAA9FC1A22ull ^ k0 ^ k1;
+    sodecl_u64 x0 = c0 + k0;
+    sodecl_u64 x1 = c1 + k1;
+    SODECL_TF_ROUND(16) SODECL_TF_ROUND(42) SODECL_TF_ROUND(12) SODECL_TF_ROUND(31)
+    x0 += k1; x1 += k2 + 1ull;
+    SODECL_TF_ROUND(16) SODECL_TF_ROUND(32) SODECL_TF_ROUND(24) SODECL_TF_ROUND(21)
+    x0 += k2; x1 += k0 + 2ull;
+    SODECL_TF_ROUND(16) SODECL_TF_ROUND(42) SODECL_TF_ROUND(12) SODECL_TF_ROUND(31)
+    x0 += k0; x1 += k1 + 3ull;
+    SODECL_TF_ROUND(16) SODECL_TF_ROUND(32) SODECL_TF_ROUND(24) SODECL_TF_ROUND(21)
+    x0 += k1; x1 += k2 + 4ull;
+    SODECL_TF_ROUND(16) SODECL_TF_ROUND(42) SODECL_TF_ROUND(12) SODECL_TF_ROUND(31)
+    x0 += k2; x1 += k0 + 5ull;
+    *o0 = x0;
+    *o1 = x1;
+}
+
+// 64-bit word -> double in the open interval (0, 1): (1 + (w >> 11)) * (2^53-1) * 2^-106,
+// the open-open 64->53 conversion of Random123's u01.h:129-131.
+SODECL_INLINE double sodecl_u01(sodecl_u64 w)
+{
+    return (double)(1ull + (w >> 11)) * 0x1.fffffffffffffp-54;
+}
+
+// The reference keeps the per-orbit counter as a double and assigns it to the
+// 64-bit counter word (stochastic_solver_caller.cl:49).  Non-negative values
+// truncate; negative values (undefined in C) take the two's complement of the
+// truncated integer, as the CPU restatement does.
+SODECL_INLINE sodecl_u64 sodecl_counter_word(double c)
+{
+    return c >= 0.0 ? (sodecl_u64)c : (sodecl_u64)(long long)c;
+}
+
+// Key and counter of the reference's SDE caller: key {0, 0xf00dcafe} (the
+// orbit id written into the key is overwritten with 0, :31 and :38), counter
+// {orbit, counter_g[orbit]}, one counter increment per Threefry call.
+#define SODECL_RNG_KEY0 0ull
+#define SODECL_RNG_KEY1 0xf00dcafeull
+
+// One Box-Muller pair scaled by sqrt(dt) (stochastic_solver_caller.cl:54-65).
+SODECL_INLINE void sodecl_noise_pair(int orbit, double counter, double dtsqrt, double* z0, double* z1)
+{
+    sodecl_u64 w0, w1;
+    sodecl_threefry2x64_20((sodecl_u64)orbit, sodecl_counter_word(counter), SODECL_RNG_KEY0, SODECL_RNG_KEY1,
+                           &w0, &w1);
+    const double u0 = sodecl_u01(w0);
+    const double u1 = sodecl_u01(w1);
+    const double r = sqrt(-2 * log(u0));
+    double s, c;
+    sincos(2 * M_PI * u1, &s, &c);
+    *z0 = (r * c) * dtsqrt;
+    *z1 = (r * s) * dtsqrt;
+}
+
+// ---------------------------------------------------------------------------
+// Implicit solvers: Newton iteration with the reference's tridiagonal solve.
+// ---------------------------------------------------------------------------
+#if SODECL_IS_IMPLICIT
+
+// Three diagonals of I + scale*J at g (ie.cl:83-92, im.cl:84-93: every entry
+// multiplied by scale then by dt, then 1 added to the diagonal).
+SODECL_INLINE void sodecl_newton_band(double* g, double* p, double scale, double* lo, double* di, double* up)
+{
+#if SODECL_HAS_JACOBIAN_BAND
+    calc_jacobian_band(g, lo, di, up, p);
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) {
+        lo[i] = lo[i] * scale * _m_dt_;
+        up[i] = up[i] * scale * _m_dt_;
+        di[i] = di[i] * scale * _m_dt_;
+        di[i] = di[i] + 1;
+    }
+#else
+    double jac[_numeq_][_numeq_];
+    calc_jacobian(g, jac, p);
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) {
+        di[i] = jac[i][i] * scale * _m_dt_;
+        di[i] = di[i] + 1;
+        up[i] = (i + 1 < _numeq_) ? jac[i][i + 1] * scale * _m_dt_ : 0.0;
+        lo[i] = (i + 1 < _numeq_) ? jac[i + 1][i] * scale * _m_dt_ : 0.0;
+    }
+#endif
+}
+
+// The reference's "Thomas" sweep (ie.cl:10-48): a Thomas solve of the
+// tridiagonal matrix with sub-diagonal a[i] = J[i][i+1] (zero for i >= N-2),
+// diagonal b[i] = J[i][i] and super-diagonal c[i] = J[i+1][i]; v is consumed.
+SODECL_INLINE void sodecl_thomas(const double* lo, const double* di, const double* up, double* v, double* x)
+{
+    double b[_numeq_];
+    b[0] = di[0];
+    SODECL_UNROLL
+    for (int i = 1; i < _numeq_; ++i) {
+        const double a = (i < _numeq_ - 2) ? up[i] : 0.0;
+        const double m = a / b[i - 1];
+        b[i] = di[i] - m * lo[i - 1];
+        v[i] = v[i] - m * v[i - 1];
+    }
+    x[_numeq_ - 1] = v[_numeq_ - 1] / b[_numeq_ - 1];
+    SODECL_UNROLL
+    for (int i = _numeq_ - 2; i >= 0; --i) {
+        x[i] = (v[i] - lo[i] * x[i + 1]) / b[i];
+    }
+}
+
+// One implicit step from y: implicit Euler (ie.cl:50-117, L1 stop test,
+// f evaluated at (t, g)) or implicit midpoint (im.cl:43-118, L2 stop test,
+// f and J evaluated at (y + g)/2).  Returns the Newton updates taken.
+SODECL_INLINE int sodecl_implicit_step(double t, double* y, double* yout, double* p)
+{
+    double g[_numeq_], r[_numeq_], lo[_numeq_], di[_numeq_], up[_numeq_], x[_numeq_];
+    int updates = 0;
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) g[i] = y[i];
+    for (int it = 0; it < SODECL_NEWTON_MAXIT; ++it) {
+#if SODECL_SOLVER == 3
+        sode_system(t, g, r, p);
+        double err = 0;
+        SODECL_UNROLL
+        for (int i = 0; i < _numeq_; ++i) {
+            r[i] = g[i] - y[i] - r[i] * _m_dt_;
+            err = err + fabs(r[i]);
+        }
+        if (err < _epsilon1_) break;
+        sodecl_newton_band(g, p, -1.00, lo, di, up);
+#else
+        double mid[_numeq_];
+        SODECL_UNROLL
+        for (int i = 0; i < _numeq_; ++i) mid[i] = (y[i] + g[i]) * 0.5;
+        sode_system(t, mid, r, p);
+        double err = 0;
+        SODECL_UNROLL
+        for (int i = 0; i < _numeq_; ++i) {
+            r[i] = g[i] - y[i] - r[i] * _m_dt_;
+            err = err + r[i] * r[i];
+        }
+        if (sqrt(err) < _epsilon1_) break;
+        sodecl_newton_band(mid, p, -0.50, lo, di, up);
+#endif
+        sodecl_thomas(lo, di, up, r, x);
+        SODECL_UNROLL
+        for (int i = 0; i < _numeq_; ++i) g[i] = g[i] - x[i];
+        ++updates;
+    }
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) yout[i] = g[i];
+    return updates;
+}
+#endif  // SODECL_IS_IMPLICIT
+
+// ---------------------------------------------------------------------------
+// One step of the selected solver, lane schedule (whole state per lane).
+// ---------------------------------------------------------------------------
+#if SODECL_SOLVER == 1
+// Euler (euler.cl:10-18)
+SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
+{
+    sode_system(t, y, yout, p);
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + yout[i] * _m_dt_;
+    return 0;
+}
+#elif SODECL_SOLVER == 2
+// Classic RK4 (rk4.cl:17-64): yout accumulates k1 + 2 k2 + 2 k3, the update
+// is y + (acc + k4) * dt / 6.00, stage times t, t+dt/2, t+dt/2, t+dt.
+SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
+{
+    double k[_numeq_], x[_numeq_];
+    sode_system(t, y, yout, p);
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) x[i] = y[i] + (_m_dt_ * 0.50) * yout[i];
+    sode_system(t + 0.50 * _m_dt_, x, k, p);
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) {
+        yout[i] = yout[i] + 2.00 * k[i];
+        x[i] = y[i] + (_m_dt_ * 0.50) * k[i];
+    }
+    sode_system(t + 0.50 * _m_dt_, x, k, p);
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) {
+        yout[i] = yout[i] + 2.00 * k[i];
+        x[i] = y[i] + _m_dt_ * k[i];
+    }
+    sode_system(t + _m_dt_, x, k, p);
+    SODECL_UNROLL
+    for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + (yout[i] + k[i]) * _m_dt_ / 6.00;
+    return 0;
+}
+#elif SODECL_IS_IMPLICIT
+SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
+{
+    return sodecl_implicit_step(t, y, yout, p);
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// Lane schedule kernel: one orbit per lane.
+//   t_state[orbits], y_state[orbits][neq] (AoS, the reference's y0 buffer),
+//   params[orbits][npar], counters[orbits] (SDE), all updated in place;
+//   traj: snapshot buffer [snap][neq][orbits] or null; snapshot s0 + s is the
+//   state before segment s, and s0 + nseg the final state when final_snap.
+//   newton: per-orbit Newton update counts (implicit solvers) or null.
+// ---------------------------------------------------------------------------
+#if SODECL_SCHEDULE == 1
+extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
+sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
+            double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
+            int orbits, int s0, int nseg, int final_snap)
+{
+    const int i = blockIdx.x * SODECL_BLOCK + threadIdx.x;
+    if (i >= orbits) return;
+    double y[_numeq_], yn[_numeq_], p[SODECL_NPAR_ALLOC];
+    SODECL_UNROLL
+    for (int m = 0; m < _numeq_; ++m) y[m] = y_state[(size_t)i * _numeq_ + m];
+    for (int m = 0; m < _numpar_; ++m) p[m] = params[(size_t)i * _numpar_ + m];
+    double t = t_state[i];
+    const size_t slab = (size_t)_numeq_ * (size_t)orbits;
+#if SODECL_IS_SDE
+    double cnt = counters[i];
+    const double dtsqrt = sqrt(_m_dt_);
+    const int calls = (int)ceilf((float)((float)(_numnoi_) * 0.5));
+    double noise[SODECL_NNOI_ALLOC];
+#endif
+    long long nu = 0;
+    for (int s = 0; s < nseg; ++s) {
+        if (traj) {
+            double* o = traj + (size_t)(s0 + s) * slab + i;
+            SODECL_UNROLL
+            for (int m = 0; m < _numeq_; ++m) o[(size_t)m * orbits] = y[m];
+        }
+        for (int it = 0; it < _numsteps_; ++it) {
+#if SODECL_IS_SDE
+            // Noise for this step (stochastic_solver_caller.cl:44-66)
+            int nc = 0;
+            for (int in = 0; in < calls; ++in) {
+                double z0, z1;
+                sodecl_noise_pair(i, cnt, dtsqrt, &z0, &z1);
+                cnt = cnt + 1;
+                if (nc < _numnoi_) noise[nc++] = z0;
+                if (nc < _numnoi_) noise[nc++] = z1;
+            }
+            // Euler-Maruyama (stochastic_euler.cl:10-22)
+            {
+                double st[_numeq_];
+                sode_system(t, y, yn, p);
+                sode_system_stoch(t, y, st, p, noise);
+                SODECL_UNROLL
+                for (int m = 0; m < _numeq_; ++m) yn[m] = y[m] + yn[m] * _m_dt_ + st[m];
+            }
+#else
+            nu += sodecl_lane_step(t, y, yn, p);
+#endif
+            t = t + _m_dt_;
+            SODECL_UNROLL
+            for (int m = 0; m < _numeq_; ++m) y[m] = yn[m];
+        }
+    }
+    if (traj && final_snap) {
+        double* o = traj + (size_t)(s0 + nseg) * slab + i;
+        SODECL_UNROLL
+        for (int m = 0; m < _numeq_; ++m) o[(size_t)m * orbits] = y[m];
+    }
+    SODECL_UNROLL
+    for (int m = 0; m < _numeq_; ++m) y_state[(size_t)i * _numeq_ + m] = y[m];
+    t_state[i] = t;
+#if SODECL_IS_SDE
+    counters[i] = cnt;
+#endif
+    if (newton) newton[i] += nu;
+}
+#endif  // SODECL_SCHEDULE == 1
+
+// ---------------------------------------------------------------------------
+// Group schedule: SODECL_GROUP lanes per orbit (explicit solvers).
+// Lane l of a group owns components l, l + G, l + 2G, ...; each stage vector
+// is written to an LDS row, one barrier publishes it, and every lane then
+// evaluates its components with the plugin's per-component entry point,
+// reading the whole stage vector from LDS (same-address reads broadcast).
+// Rows are double-buffered so one barrier per stage suffices.
+// ---------------------------------------------------------------------------
+#if SODECL_SCHEDULE == 2 && SODECL_HAS_COMPONENT && !SODECL_IS_IMPLICIT && (!SODECL_IS_SDE || SODECL_HAS_STOCH_COMPONENT)
+
+#define SODECL_G SODECL_GROUP
+#define SODECL_CPL ((_numeq_ + SODECL_G - 1) / SODECL_G)
+#define SODECL_OPB (SODECL_BLOCK / SODECL_G)
+#define SODECL_XSTRIDE (_numeq_ | 1)
+
+extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
+sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
+             double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
+             int orbits, int s0, int nseg, int final_snap)
+{
+    __shared__ double sh_x[2][SODECL_OPB][SODECL_XSTRIDE];
+    __shared__ double sh_p[SODECL_OPB][SODECL_NPAR_ALLOC];
+#if SODECL_IS_SDE
+    __shared__ double sh_n[2][SODECL_OPB][SODECL_NNOI_ALLOC];
+#endif
+    const int lane = threadIdx.x % SODECL_G;
+    const int slot = threadIdx.x / SODECL_G;
+    const int orbit = blockIdx.x * SODECL_OPB + slot;
+    const bool active = orbit < orbits;
+    (void)newton;
+
+    for (int k = lane; k < _numpar_; k += SODECL_G)
+        sh_p[slot][k] = active ? params[(size_t)orbit * _numpar_ + k] : 0.0;
+    double y[SODECL_CPL];
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        y[c] = (active && comp < _numeq_) ? y_state[(size_t)orbit * _numeq_ + comp] : 0.0;
+    }
+    double t = active ? t_state[orbit] : 0.0;
+    double* const pp = sh_p[slot];
+    const size_t slab = (size_t)_numeq_ * (size_t)orbits;
+#if SODECL_IS_SDE
+    double cnt = active ? counters[orbit] : 0.0;
+    const double dtsqrt = sqrt(_m_dt_);
+    const int calls = (int)ceilf((float)((float)(_numnoi_) * 0.5));
+#endif
+    int buf = 0;
+
+#define SODECL_PUBLISH(VALS)                                                  \
+    _Pragma("unroll") for (int c = 0; c < SODECL_CPL; ++c) {                  \
+        const int comp = lane + c * SODECL_G;                                 \
+        if (comp < _numeq_) sh_x[buf][slot][comp] = VALS[c];                  \
+    }
+
+    for (int s = 0; s < nseg; ++s) {
+        if (traj && active) {
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                if (comp < _numeq_) traj[(size_t)(s0 + s) * slab + (size_t)comp * orbits + orbit] = y[c];
+            }
+        }
+        for (int it = 0; it < _numsteps_; ++it) {
+#if SODECL_SOLVER == 1
+            // Euler (euler.cl:10-18)
+            SODECL_PUBLISH(y)
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                if (comp < _numeq_) {
+                    const double f = sode_system_i(comp, t, sh_x[buf][slot], pp);
+                    y[c] = y[c] + f * _m_dt_;
+                }
+            }
+            buf ^= 1;
+#elif SODECL_SOLVER == 2
+            // Classic RK4 (rk4.cl:17-64), one LDS stage row per evaluation
+            double acc[SODECL_CPL], xn[SODECL_CPL];
+            SODECL_PUBLISH(y)
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                const double f = comp < _numeq_ ? sode_system_i(comp, t, sh_x[buf][slot], pp) : 0.0;
+                acc[c] = f;
+                xn[c] = y[c] + (_m_dt_ * 0.50) * f;
+            }
+            buf ^= 1;
+            SODECL_PUBLISH(xn)
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, sh_x[buf][slot], pp) : 0.0;
+                acc[c] = acc[c] + 2.00 * f;
+                xn[c] = y[c] + (_m_dt_ * 0.50) * f;
+            }
+            buf ^= 1;
+            SODECL_PUBLISH(xn)
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, sh_x[buf][slot], pp) : 0.0;
+                acc[c] = acc[c] + 2.00 * f;
+                xn[c] = y[c] + _m_dt_ * f;
+            }
+            buf ^= 1;
+            SODECL_PUBLISH(xn)
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                const double f = comp < _numeq_ ? sode_system_i(comp, t + _m_dt_, sh_x[buf][slot], pp) : 0.0;
+                y[c] = y[c] + (acc[c] + f) * _m_dt_ / 6.00;
+            }
+            buf ^= 1;
+#elif SODECL_IS_SDE
+            // Noise: Threefry call k of this step goes to lane k of the group;
+            // the counter of call k is counter + k, as the reference's
+            // sequential +1 per call (stochastic_solver_caller.cl:44-66).
+            for (int k = lane; k < calls; k += SODECL_G) {
+                double z0, z1;
+                sodecl_noise_pair(orbit, cnt + (double)k, dtsqrt, &z0, &z1);
+                if (2 * k < _numnoi_) sh_n[buf][slot][2 * k] = z0;
+                if (2 * k + 1 < _numnoi_) sh_n[buf][slot][2 * k + 1] = z1;
+            }
+            cnt = cnt + (double)calls;
+            SODECL_PUBLISH(y)
+            __syncthreads();
+            // Euler-Maruyama (stochastic_euler.cl:10-22)
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                if (comp < _numeq_) {
+                    const double f = sode_system_i(comp, t, sh_x[buf][slot], pp);
+                    const double g = sode_system_stoch_i(comp, t, sh_x[buf][slot], pp, sh_n[buf][slot]);
+                    y[c] = y[c] + f * _m_dt_ + g;
+                }
+            }
+            buf ^= 1;
+#endif
+            t = t + _m_dt_;
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int c = 0; c < SODECL_CPL; ++c) {
+            const int comp = lane + c * SODECL_G;
+            if (comp < _numeq_) {
+                if (traj && final_snap)
+                    traj[(size_t)(s0 + nseg) * slab + (size_t)comp * orbits + orbit] = y[c];
+                y_state[(size_t)orbit * _numeq_ + comp] = y[c];
+            }
+        }
+        if (lane == 0) {
+            t_state[orbit] = t;
+#if SODECL_IS_SDE
+            counters[orbit] = cnt;
+#endif
+        }
+    }
+#undef SODECL_PUBLISH
+}
+#endif  // group schedule

@@ -1,0 +1,275 @@
+// Program assembly and hipRTC compilation.
+//
+// The reference builds one OpenCL program per call by concatenating a macro
+// prelude, the user system source, the solver source and the caller kernel
+// (src/sodeclmgr.cpp:374-503) and compiles it with the OpenCL runtime
+// (:559-622).  Here the same macro contract is emitted, the user's OpenCL-C
+// style functions are compiled as host+device functions, and the MI355X
+// integrator templates (device/integrators.hip, embedded in this library)
+// follow; hipRTC compiles the text for gfx950.  Code objects are cached in
+// memory and on disk, keyed by a hash of the text, the options and the
+// target, so repeated calls skip the compiler.
+#include <dlfcn.h>
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <regex>
+#include <sstream>
+
+#include "sodecl_internal.hpp"
+
+namespace {
+#include "_embedded_device.inc"   // kIntegratorsSource, generated from device/*.hip
+}
+
+namespace sodecl_amd {
+
+std::string strip_comments(const std::string& s)
+{
+    std::string out;
+    out.reserve(s.size());
+    size_t i = 0, n = s.size();
+    while (i < n) {
+        if (s[i] == '/' && i + 1 < n && s[i + 1] == '/') {
+            while (i < n && s[i] != '\n') ++i;
+        } else if (s[i] == '/' && i + 1 < n && s[i + 1] == '*') {
+            i += 2;
+            while (i + 1 < n && !(s[i] == '*' && s[i + 1] == '/')) ++i;
+            i += 2;
+            out.push_back(' ');
+        } else if (s[i] == '"') {
+            out.push_back(s[i++]);
+            while (i < n && s[i] != '"') {
+                if (s[i] == '\\' && i + 1 < n) out.push_back(s[i++]);
+                out.push_back(s[i++]);
+            }
+            if (i < n) out.push_back(s[i++]);
+        } else {
+            out.push_back(s[i++]);
+        }
+    }
+    return out;
+}
+
+PluginFeatures detect_features(const std::string& source)
+{
+    const std::string s = strip_comments(source);
+    auto defines = [&](const char* name) {
+        // a function definition or declaration: name '(' ... at file scope
+        std::regex re(std::string("(^|[^A-Za-z0-9_])") + name + "\\s*\\(");
+        return std::regex_search(s, re);
+    };
+    PluginFeatures f;
+    f.sode_system = defines("sode_system");
+    f.sode_system_stoch = defines("sode_system_stoch");
+    f.calc_jacobian = defines("calc_jacobian");
+    f.component = defines("sode_system_i");
+    f.stoch_component = defines("sode_system_stoch_i");
+    f.jacobian_band = defines("calc_jacobian_band");
+    f.jacobian_band_i = defines("calc_jacobian_band_i");
+    f.vjp_group = defines("sode_system_vjp_group");
+    return f;
+}
+
+std::string dt_literal(double dt)
+{
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%g", dt);
+    std::string s(buf);
+    // a literal such as "1" must still be a double in the kernel
+    if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
+    return s;
+}
+
+long num_launches(double tspan, double dt, int ksteps)
+{
+    // sodeclmgr.cpp:951: for (int j = 0; j < tspan/(dt*ksteps); j++)
+    const double x = tspan / (dt * ksteps);
+    if (!(x > 0)) return 0;
+    if (x > 2147483647.0) return -1;
+    long n = (long)std::ceil(x);
+    while (n > 0 && !((double)(n - 1) < x)) --n;
+    while ((double)n < x) ++n;
+    return n;
+}
+
+std::string assemble_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f)
+{
+    std::ostringstream o;
+    o << "// SODECL program assembled for hipRTC\n";
+    // The reference's macro contract (sodeclmgr.cpp:389-430)
+    o << "#define _numnoi_ " << k.nnoi << "\n";
+    o << "#define _listsize_ " << k.orbits << "\n";
+    o << "#define _numsteps_ " << k.ksteps << "\n";
+    o << "#define _m_dt_ " << dt_literal(k.dt) << "\n";
+    o << "#define _numeq_ " << k.neq << "\n";
+    o << "#define _numpar_ " << k.npar << "\n";
+    if (k.solver == SODECL_IMPLICIT_EULER || k.solver == SODECL_IMPLICIT_MIDPOINT) o << "#define _epsilon1_ 1e-07\n";
+    // MI355X build configuration
+    o << "#define SODECL_HIP 1\n";
+    o << "#define SODECL_SOLVER " << k.solver << "\n";
+    o << "#define SODECL_SCHEDULE " << k.schedule << "\n";
+    o << "#define SODECL_GROUP " << k.group << "\n";
+    o << "#define SODECL_BLOCK " << k.block << "\n";
+    o << "#define SODECL_NEWTON_MAXIT " << k.newton_max_iter << "\n";
+    o << "#define SODECL_HAS_COMPONENT " << (f.component ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_STOCH_COMPONENT " << (f.stoch_component ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN_BAND " << (f.jacobian_band ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN_BAND_I " << (f.jacobian_band_i ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN " << (f.calc_jacobian ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_VJP_GROUP " << (f.vjp_group ? 1 : 0) << "\n";
+    o << k.extra_defines;
+    // OpenCL C spellings a system source may use
+    for (const char* kw : {"__kernel", "__global", "__private"})
+        o << "#ifndef " << kw << "\n#define " << kw << "\n#endif\n";
+    o << "#ifndef __local\n#define __local\n#endif\n#ifndef __constant\n#define __constant const\n#endif\n";
+    o << "typedef unsigned long ulong;\ntypedef unsigned int uint;\ntypedef unsigned short ushort;\n"
+         "typedef unsigned char uchar;\n";
+    o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
+    o << "#ifndef M_PI_F\n#define M_PI_F 3.14159265358979323846f\n#endif\n";
+    // The plugin's functions become host+device functions.
+    o << "#pragma clang force_cuda_host_device begin\n";
+    o << "#line 1 \"system.cl\"\n";
+    o << system_source << "\n";
+    o << "#pragma clang force_cuda_host_device end\n";
+    o << "#line 1 \"integrators.hip\"\n";
+    o << kIntegratorsSource << "\n";
+    return o.str();
+}
+
+hipFunction_t Program::get(const std::string& name)
+{
+    auto it = functions.find(name);
+    if (it != functions.end()) return it->second;
+    hipFunction_t fn = nullptr;
+    if (module && hipModuleGetFunction(&fn, module, name.c_str()) != hipSuccess) fn = nullptr;
+    functions[name] = fn;
+    return fn;
+}
+
+namespace {
+
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ull)
+{
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string cache_dir()
+{
+    if (const char* e = std::getenv("SODECL_AMD_CACHE_DIR")) return e;
+    Dl_info info;
+    if (dladdr((void*)&fnv1a, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        auto pos = p.rfind('/');
+        if (pos != std::string::npos) return p.substr(0, pos) + "/_jit_cache";
+    }
+    return "/tmp/sodecl_amd_jit_cache";
+}
+
+std::mutex g_cache_mu;
+std::map<uint64_t, std::vector<char>> g_mem_cache;
+
+bool read_file(const std::string& path, std::vector<char>& data)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    data.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return !data.empty();
+}
+
+void write_file_atomic(const std::string& dir, const std::string& name, const std::vector<char>& data)
+{
+    mkdir(dir.c_str(), 0755);
+    std::string tmp = dir + "/." + name + "." + std::to_string(getpid());
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return;
+        f.write(data.data(), (std::streamsize)data.size());
+    }
+    std::rename(tmp.c_str(), (dir + "/" + name).c_str());
+}
+
+}  // namespace
+
+std::string device_arch(int device)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "gfx950";
+    std::string a(prop.gcnArchName);
+    auto pos = a.find(':');
+    if (pos != std::string::npos) a = a.substr(0, pos);
+    return a.empty() ? "gfx950" : a;
+}
+
+bool compile_program(const std::string& text, const std::string& arch, int fp_contract, std::vector<char>& code,
+                     std::string& log)
+{
+    int vmaj = 0, vmin = 0;
+    hiprtcVersion(&vmaj, &vmin);
+    std::vector<std::string> opts = {"--gpu-architecture=" + arch, "-O3", "-std=c++17",
+                                     fp_contract ? "-ffp-contract=off" : "-ffp-contract=fast",
+                                     // no packed FP32 instructions in any generated code
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    std::string okey;
+    for (auto& s : opts) okey += s + " ";
+    okey += "hiprtc" + std::to_string(vmaj) + "." + std::to_string(vmin);
+    const uint64_t h = fnv1a(okey, fnv1a(text));
+    char hname[40];
+    std::snprintf(hname, sizeof hname, "%016llx.hsaco", (unsigned long long)h);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_mem_cache.find(h);
+        if (it != g_mem_cache.end()) {
+            code = it->second;
+            return true;
+        }
+    }
+    const std::string dir = cache_dir();
+    if (!std::getenv("SODECL_AMD_NO_DISK_CACHE") && read_file(dir + "/" + hname, code)) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_mem_cache[h] = code;
+        return true;
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, text.c_str(), "sodecl_program.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    std::vector<const char*> copts;
+    for (auto& s : opts) copts.push_back(s.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    size_t lsz = 0;
+    hiprtcGetProgramLogSize(prog, &lsz);
+    log.assign(lsz, '\0');
+    if (lsz) hiprtcGetProgramLog(prog, &log[0]);
+    while (!log.empty() && log.back() == '\0') log.pop_back();
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t csz = 0;
+    hiprtcGetCodeSize(prog, &csz);
+    code.resize(csz);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_mem_cache[h] = code;
+    }
+    if (!std::getenv("SODECL_AMD_NO_DISK_CACHE")) write_file_atomic(dir, hname, code);
+    return true;
+}
+
+}  // namespace sodecl_amd

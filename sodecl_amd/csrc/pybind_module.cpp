@@ -1,0 +1,258 @@
+// Python module sodecl_interface: the reference's pybind11 entry point
+// (interfaces/python/sodecl.cpp:16-101 of avramidis/sodecl) over the C ABI,
+// plus a persistent Session (program compiled once, reused across calls) and
+// a device-pointer entry for PyTorch tensors.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/sodecl.h"
+
+namespace py = pybind11;
+using darr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+namespace {
+
+void apply_options(sodecl_options& o, const py::dict& d)
+{
+    for (auto kv : d) {
+        std::string k = py::str(kv.first);
+        if (k == "schedule") {
+            if (py::isinstance<py::str>(kv.second)) {
+                std::string v = py::str(kv.second);
+                o.schedule = v == "lane" ? SODECL_SCHEDULE_LANE
+                             : v == "group" ? SODECL_SCHEDULE_GROUP
+                             : v == "plain" ? SODECL_SCHEDULE_PLAIN
+                                            : SODECL_SCHEDULE_AUTO;
+            } else {
+                o.schedule = kv.second.cast<int>();
+            }
+        } else if (k == "group_size") o.group_size = kv.second.cast<int>();
+        else if (k == "block_size") o.block_size = kv.second.cast<int>();
+        else if (k == "newton_max_iter") o.newton_max_iter = kv.second.cast<int>();
+        else if (k == "segments_per_launch") o.segments_per_launch = kv.second.cast<int>();
+        else if (k == "use_graph") o.use_graph = kv.second.cast<int>();
+        else if (k == "autotune") o.autotune = kv.second.cast<int>();
+        else if (k == "fp_contract") o.fp_contract = kv.second.cast<int>();
+        else if (k == "verbose") o.verbose = kv.second.cast<int>();
+        else if (k == "traj_budget_bytes") o.traj_budget_bytes = kv.second.cast<size_t>();
+        else throw std::invalid_argument("unknown option " + k);
+    }
+}
+
+// sodecl.cpp:16-95
+py::array_t<double> sodeclcall(darr a_t0, darr a_y0, darr a_params, int a_platform, int a_device, std::string a_system,
+                               int a_solver_type, int a_orbits, int a_equats, int a_nparams, int a_nnoi, double a_dt,
+                               double a_tspan, int a_ksteps, int a_local_group_size)
+{
+    if (a_solver_type < 0 || a_solver_type > 4) {
+        std::cout << "Unknown SODE solver selection." << std::endl;
+        return py::array_t<double>(0);
+    }
+    if ((size_t)a_y0.size() < (size_t)a_orbits * a_equats || (size_t)a_params.size() < (size_t)a_orbits * a_nparams ||
+        (size_t)a_t0.size() < (size_t)a_orbits)
+        throw std::invalid_argument("t0, y0 or params is smaller than orbits x size");
+    sodecl_ctx* ctx = nullptr;
+    if (!sodecl_create(&ctx, "kernels", a_system.c_str(), a_solver_type, a_dt, a_tspan, a_ksteps, a_equats, a_nparams,
+                       a_nnoi, a_orbits, SODECL_OUTPUT_ARRAY)) {
+        std::cout << sodecl_last_error(ctx) << std::endl;
+        sodecl_destroy(ctx);
+        return py::array_t<double>(0);
+    }
+    if (!sodecl_choose_device(ctx, (uint32_t)a_platform, SODECL_DEVICE_TYPE_ALL, (uint32_t)a_device))
+        std::cerr << "Error selecting OpenCL device!" << std::endl;
+    sodecl_set_t0(ctx, a_t0.data());
+    sodecl_set_y0(ctx, a_y0.data());
+    sodecl_set_params(ctx, a_params.data());
+    sodecl_set_local_group_size(ctx, a_local_group_size);
+    const long launches = sodecl_num_launches(a_tspan, a_dt, a_ksteps);
+    const size_t total = (size_t)(launches + 1) * (size_t)a_equats * (size_t)a_orbits;
+    py::array_t<double> out(total);
+    int ok;
+    {
+        py::gil_scoped_release nogil;
+        ok = sodecl_setup(ctx);
+        if (ok) {
+            sodecl_set_output_buffer(ctx, out.mutable_data(), total);
+            ok = sodecl_run(ctx);
+        }
+    }
+    if (!ok) {
+        std::cout << sodecl_last_error(ctx) << std::endl;
+        sodecl_destroy(ctx);
+        return py::array_t<double>(0);
+    }
+    sodecl_destroy(ctx);
+    return out;
+}
+
+// A manager object kept alive across calls: the program is compiled once.
+class Session {
+public:
+    Session(const std::string& system, int solver, int orbits, int neq, int npar, int nnoi, double dt, double tspan,
+            int ksteps, int device, py::dict options, bool system_is_source)
+        : orbits_(orbits), neq_(neq), npar_(npar), nnoi_(nnoi), dt_(dt), tspan_(tspan), ksteps_(ksteps)
+    {
+        int ok = system_is_source
+                     ? sodecl_create_from_source(&ctx_, system.c_str(), solver, dt, tspan, ksteps, neq, npar, nnoi,
+                                                 orbits, SODECL_OUTPUT_ARRAY)
+                     : sodecl_create(&ctx_, "kernels", system.c_str(), solver, dt, tspan, ksteps, neq, npar, nnoi,
+                                     orbits, SODECL_OUTPUT_ARRAY);
+        if (!ok) throw std::runtime_error(sodecl_last_error(ctx_));
+        sodecl_default_options(&opts_);
+        apply_options(opts_, options);
+        if (!sodecl_set_options(ctx_, &opts_)) throw std::runtime_error(sodecl_last_error(ctx_));
+        device_ = device;
+        compiled_only_ = false;
+    }
+    ~Session() { sodecl_destroy(ctx_); }
+
+    long launches() const { return sodecl_num_launches(tspan_, dt_, ksteps_); }
+    long nsnap() const { return launches() + 1; }
+
+    void setup(darr y0, darr params, py::object t0, py::object counters)
+    {
+        check(y0.size() >= (py::ssize_t)orbits_ * neq_ && params.size() >= (py::ssize_t)orbits_ * npar_,
+              "y0/params too small");
+        if (!sodecl_choose_device(ctx_, 0, SODECL_DEVICE_TYPE_ALL, (uint32_t)device_)) fail();
+        y0_ = y0;
+        params_ = params;
+        sodecl_set_y0(ctx_, y0_.data());
+        sodecl_set_params(ctx_, params_.data());
+        if (!t0.is_none()) {
+            t0_ = darr::ensure(t0);
+            check(t0_.size() >= orbits_, "t0 too small");
+            sodecl_set_t0(ctx_, t0_.data());
+        } else {
+            sodecl_set_t0(ctx_, nullptr);
+        }
+        if (!counters.is_none()) {
+            darr c = darr::ensure(counters);
+            check(c.size() >= orbits_, "counters too small");
+            if (!sodecl_set_rng_counters(ctx_, c.data())) fail();
+        }
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_setup(ctx_);
+        }
+        if (!ok) fail();
+    }
+
+    // Host run: returns the flat [snap][eq][orbit] output of this run.
+    py::array_t<double> run()
+    {
+        const size_t total = (size_t)nsnap() * neq_ * orbits_;
+        py::array_t<double> out(total);
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            sodecl_set_output_buffer(ctx_, out.mutable_data(), total);
+            ok = sodecl_run(ctx_);
+            sodecl_set_output_buffer(ctx_, nullptr, 0);
+        }
+        if (!ok) fail();
+        return out;
+    }
+
+    // Device run on caller memory (integer device pointers, e.g. tensor.data_ptr()).
+    void run_device(uintptr_t t0, uintptr_t y0, uintptr_t params, uintptr_t counters, uintptr_t traj, uintptr_t stream)
+    {
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_run_device(ctx_, (double*)t0, (double*)y0, (const double*)params, (double*)counters,
+                                   (double*)traj, (void*)stream);
+        }
+        if (!ok) fail();
+    }
+
+    py::array_t<double> get_counters()
+    {
+        py::array_t<double> c(orbits_);
+        if (!sodecl_get_rng_counters(ctx_, c.mutable_data())) fail();
+        return c;
+    }
+    py::array_t<long long> newton_updates()
+    {
+        py::array_t<long long> c(orbits_);
+        if (!sodecl_get_newton_updates(ctx_, c.mutable_data())) fail();
+        return c;
+    }
+    py::tuple launch_config()
+    {
+        int s = 0, g = 0, b = 0;
+        sodecl_get_launch_config(ctx_, &s, &g, &b);
+        return py::make_tuple(s, g, b);
+    }
+    double last_kernel_ms() const { return sodecl_last_kernel_ms(ctx_); }
+    double last_runtime() const { return sodecl_last_runtime(ctx_); }
+    size_t compile_only(const std::string& arch)
+    {
+        size_t n = sodecl_compile_only(ctx_, arch.c_str());
+        if (!n) fail();
+        return n;
+    }
+    std::string program_source() { return sodecl_program_source(ctx_); }
+    void set_options(py::dict options)
+    {
+        apply_options(opts_, options);
+        if (!sodecl_set_options(ctx_, &opts_)) fail();
+    }
+    uintptr_t handle() const { return (uintptr_t)ctx_; }
+
+private:
+    void check(bool cond, const char* msg)
+    {
+        if (!cond) throw std::invalid_argument(msg);
+    }
+    [[noreturn]] void fail() { throw std::runtime_error(sodecl_last_error(ctx_)); }
+
+    sodecl_ctx* ctx_ = nullptr;
+    sodecl_options opts_{};
+    int orbits_, neq_, npar_, nnoi_;
+    double dt_, tspan_;
+    int ksteps_;
+    int device_ = 0;
+    bool compiled_only_;
+    darr y0_, params_, t0_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(sodecl_interface, m)
+{
+    m.doc() = "sodecl plugin (MI355X)";
+    m.def("sodeclcall", &sodeclcall, "A function that integrates a SDE/ODE system.", py::arg("t0"), py::arg("y0"),
+          py::arg("params"), py::arg("platform"), py::arg("device"), py::arg("system"), py::arg("solver_type"),
+          py::arg("orbits"), py::arg("equats"), py::arg("nparams"), py::arg("nnoi"), py::arg("dt"), py::arg("tspan"),
+          py::arg("ksteps"), py::arg("local_group_size"));
+    m.def("num_launches", &sodecl_num_launches);
+    m.attr("ABI_VERSION") = SODECL_AMD_ABI_VERSION;
+    py::class_<Session>(m, "Session")
+        .def(py::init<const std::string&, int, int, int, int, int, double, double, int, int, py::dict, bool>(),
+             py::arg("system"), py::arg("solver"), py::arg("orbits"), py::arg("neq"), py::arg("npar"),
+             py::arg("nnoi"), py::arg("dt"), py::arg("tspan"), py::arg("ksteps"), py::arg("device") = 0,
+             py::arg("options") = py::dict(), py::arg("system_is_source") = false)
+        .def("setup", &Session::setup, py::arg("y0"), py::arg("params"), py::arg("t0") = py::none(),
+             py::arg("counters") = py::none())
+        .def("run", &Session::run)
+        .def("run_device", &Session::run_device, py::arg("t0"), py::arg("y0"), py::arg("params"),
+             py::arg("counters"), py::arg("traj"), py::arg("stream") = 0)
+        .def("counters", &Session::get_counters)
+        .def("newton_updates", &Session::newton_updates)
+        .def("launch_config", &Session::launch_config)
+        .def("last_kernel_ms", &Session::last_kernel_ms)
+        .def("last_runtime", &Session::last_runtime)
+        .def("compile_only", &Session::compile_only, py::arg("arch") = "gfx950")
+        .def("program_source", &Session::program_source)
+        .def("set_options", &Session::set_options)
+        .def("handle", &Session::handle)
+        .def_property_readonly("nsnap", &Session::nsnap)
+        .def_property_readonly("launches", &Session::launches);
+}

@@ -1,0 +1,794 @@
+// C ABI of libsodecl_amd.so (include/sodecl.h): the MI355X counterpart of the
+// reference's host manager sodecl::sodeclmgr (src/sodeclmgr.cpp).
+//
+// Differences in design from the reference's OpenCL host loop
+// (sodeclmgr.cpp:921-1035), results unchanged:
+//   * one kernel launch runs many ksteps segments and writes every snapshot
+//     straight into a device trajectory buffer in the reference's output
+//     order [snapshot][equation][orbit]; the host copies it once, instead of a
+//     blocking read-back and a host transposition before every launch;
+//   * when the trajectory exceeds the device buffer budget the run is cut
+//     into chunks whose copies overlap the next chunk's kernel (two buffers,
+//     a copy stream);
+//   * the program is compiled once per configuration and cached.
+// The reference's launch design is kept as SODECL_SCHEDULE_PLAIN, the
+// baseline the fast schedules are measured against.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <random>
+#include <sstream>
+
+#include "sodecl_internal.hpp"
+
+using sodecl_amd::fail;
+using sodecl_amd::hip_ok;
+
+namespace sodecl_amd {
+
+int fail(sodecl_ctx* ctx, const std::string& msg)
+{
+    if (ctx) ctx->last_error = msg;
+    if (ctx && ctx->opts.verbose) std::cerr << "sodecl: " << msg << std::endl;
+    return 0;
+}
+
+bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return true;
+    fail(ctx, std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+static ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra)
+{
+    ProgramKey k;
+    k.solver = ctx->solver;
+    k.neq = ctx->neq;
+    k.npar = ctx->npar;
+    k.nnoi = ctx->nnoi;
+    k.orbits = ctx->orbits;
+    k.ksteps = ctx->ksteps;
+    k.dt = ctx->dt;
+    k.schedule = schedule;
+    k.group = group;
+    k.block = block;
+    k.newton_max_iter = ctx->opts.newton_max_iter > 0 ? ctx->opts.newton_max_iter : 1000;
+    k.fp_contract = ctx->opts.fp_contract;
+    k.extra_defines = extra;
+    return k;
+}
+
+std::shared_ptr<Program> get_program(sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra)
+{
+    std::ostringstream name;
+    name << schedule << ":" << group << ":" << block << ":" << extra;
+    auto it = ctx->programs.find(name.str());
+    if (it != ctx->programs.end()) return it->second;
+    ProgramKey key = make_key(ctx, schedule, group, block, extra);
+    std::string text = assemble_program(ctx->system_source, key, ctx->features);
+    ctx->last_program = text;
+    auto prog = std::make_shared<Program>();
+    std::string log;
+    if (!compile_program(text, device_arch(ctx->device), key.fp_contract, prog->code, log)) {
+        // the reference prints the build log on failure (sodeclmgr.cpp:606-619)
+        fail(ctx, "Failed to build program executable:\n" + log);
+        return nullptr;
+    }
+    if (!hip_ok(ctx, hipSetDevice(ctx->device), "hipSetDevice")) return nullptr;
+    if (!hip_ok(ctx, hipModuleLoadData(&prog->module, prog->code.data()), "hipModuleLoadData")) return nullptr;
+    prog->device = ctx->device;
+    ctx->programs[name.str()] = prog;
+    return prog;
+}
+
+}  // namespace sodecl_amd
+
+using namespace sodecl_amd;
+
+namespace {
+
+bool read_text(const std::string& path, std::string& out)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+int init_ctx(sodecl_ctx* c, int solver, double dt, double tspan, int ksteps, int neq, int npar, int nnoi,
+             int orbits, int output_type)
+{
+    c->solver = solver;
+    c->dt = dt;
+    c->tspan = tspan;
+    c->ksteps = ksteps;
+    c->neq = neq;
+    c->npar = npar;
+    c->nnoi = nnoi;
+    c->orbits = orbits;
+    c->output_type = output_type;
+    sodecl_default_options(&c->opts);
+    if (solver < 0 || solver > 4) return fail(c, "Unknown SODE solver selection.");
+    if (neq <= 0 || npar < 0 || nnoi < 0 || orbits <= 0 || ksteps <= 0)
+        return fail(c, "invalid sizes (neq, npar, nnoi, orbits, ksteps)");
+    if (!(dt > 0) || !(tspan >= 0)) return fail(c, "invalid dt or tspan");
+    if (output_type < 0 || output_type > 2) return fail(c, "invalid output type");
+    // Per-orbit RNG counters for SDE runs: random, like the reference's
+    // m_rcounter (sodeclmgr.cpp:48-59), but non-negative whole numbers below
+    // 1e11 so the counter word is well defined.
+    if (nnoi > 0) {
+        std::random_device rd;
+        std::mt19937_64 gen(((uint64_t)rd() << 32) ^ rd());
+        std::uniform_int_distribution<long long> unif(0, 100000000000LL);
+        c->counters.resize(orbits);
+        for (auto& v : c->counters) v = (double)unif(gen);
+    }
+    return 1;
+}
+
+void free_device(sodecl_ctx* c)
+{
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    for (auto& kv : c->programs)
+        if (kv.second->module) hipModuleUnload(kv.second->module);
+    c->programs.clear();
+    hipFree(c->d_t);
+    hipFree(c->d_y);
+    hipFree(c->d_params);
+    hipFree(c->d_counters);
+    hipFree(c->d_newton);
+    hipFree(c->d_traj[0]);
+    hipFree(c->d_traj[1]);
+    hipHostFree(c->h_stage[0]);
+    hipHostFree(c->h_stage[1]);
+    if (c->ev_start) hipEventDestroy(c->ev_start);
+    if (c->ev_stop) hipEventDestroy(c->ev_stop);
+    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->copy_stream) hipStreamDestroy(c->copy_stream);
+    c->d_t = c->d_y = c->d_params = c->d_counters = nullptr;
+    c->d_newton = nullptr;
+    c->d_traj[0] = c->d_traj[1] = nullptr;
+    c->h_stage[0] = c->h_stage[1] = nullptr;
+    c->ev_start = c->ev_stop = nullptr;
+    c->stream = c->copy_stream = nullptr;
+    c->traj_capacity = c->stage_capacity = 0;
+    c->ready = false;
+}
+
+// Launch configuration for this context.
+struct LaunchCfg {
+    int schedule;   // 1 lane, 2 group
+    int group;
+    int block;
+};
+
+bool group_possible(const sodecl_ctx* c)
+{
+    const bool implicit = c->solver == SODECL_IMPLICIT_EULER || c->solver == SODECL_IMPLICIT_MIDPOINT;
+    if (implicit) return false;
+    if (!c->features.component) return false;
+    if (c->solver == SODECL_STOCHASTIC_EULER && !c->features.stoch_component) return false;
+    return true;
+}
+
+int default_group(int neq)
+{
+    int g = 1;
+    while (g < neq && g < 64) g <<= 1;
+    return std::max(g, 4);
+}
+
+LaunchCfg choose_config(const sodecl_ctx* c)
+{
+    LaunchCfg cfg{1, 64, 128};
+    int sched = c->opts.schedule;
+    if (sched == SODECL_SCHEDULE_AUTO) sched = (group_possible(c) && c->neq >= 8) ? 2 : 1;
+    if (sched == SODECL_SCHEDULE_GROUP && !group_possible(c)) sched = 1;
+    if (sched == SODECL_SCHEDULE_PLAIN) sched = 1;
+    cfg.schedule = sched;
+    if (sched == 2) {
+        int g = c->opts.group_size > 0 ? c->opts.group_size : default_group(c->neq);
+        g = std::min(std::max(g, 1), 64);
+        int p2 = 1;
+        while (p2 < g) p2 <<= 1;
+        cfg.group = p2;
+        int b = c->opts.block_size > 0 ? c->opts.block_size : 256;
+        b = std::max(b, cfg.group);
+        b = (b / cfg.group) * cfg.group;
+        cfg.block = std::min(b, 1024);
+    } else {
+        int b = c->opts.block_size > 0 ? c->opts.block_size : (c->local_group_size > 0 ? c->local_group_size : 128);
+        cfg.block = std::min(std::max(b, 1), 1024);
+        cfg.group = 1;
+    }
+    return cfg;
+}
+
+// Enqueues one launch of the chosen schedule: segments [s0, s0 + nseg) of the
+// trajectory buffer traj (snapshot rows), final snapshot when final_snap.
+int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const double* p, double* cnt, double* traj,
+            long long* newton, int s0, int nseg, int final_snap, hipStream_t stream)
+{
+    auto prog = get_program(c, cfg.schedule, cfg.group, cfg.block);
+    if (!prog) return 0;
+    const char* kname = cfg.schedule == 2 ? "sodecl_group" : "sodecl_lane";
+    hipFunction_t fn = prog->get(kname);
+    if (!fn) return fail(c, std::string("kernel ") + kname + " missing from the program");
+    int orbits = c->orbits;
+    unsigned grid;
+    if (cfg.schedule == 2) {
+        const int opb = cfg.block / cfg.group;
+        grid = (unsigned)((orbits + opb - 1) / opb);
+    } else {
+        grid = (unsigned)((orbits + cfg.block - 1) / cfg.block);
+    }
+    void* args[] = {&t, &y, (void*)&p, &cnt, &traj, &newton, &orbits, &s0, &nseg, &final_snap};
+    if (!hip_ok(c, hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)cfg.block, 1, 1, 0, stream, args, nullptr),
+                "hipModuleLaunchKernel"))
+        return 0;
+    c->used_schedule = cfg.schedule;
+    c->used_group = cfg.group;
+    c->used_block = cfg.block;
+    return 1;
+}
+
+int ensure_traj(sodecl_ctx* c, size_t doubles, int nbuf)
+{
+    if (c->traj_capacity >= doubles && (nbuf < 2 || c->d_traj[1])) return 1;
+    hipFree(c->d_traj[0]);
+    hipFree(c->d_traj[1]);
+    c->d_traj[0] = c->d_traj[1] = nullptr;
+    c->traj_capacity = 0;
+    for (int b = 0; b < nbuf; ++b)
+        if (!hip_ok(c, hipMalloc(&c->d_traj[b], doubles * sizeof(double)), "hipMalloc(trajectory)")) return 0;
+    c->traj_capacity = doubles;
+    return 1;
+}
+
+int ensure_stage(sodecl_ctx* c, size_t doubles)
+{
+    if (c->stage_capacity >= doubles) return 1;
+    hipHostFree(c->h_stage[0]);
+    hipHostFree(c->h_stage[1]);
+    c->h_stage[0] = c->h_stage[1] = nullptr;
+    c->stage_capacity = 0;
+    for (int b = 0; b < 2; ++b)
+        if (!hip_ok(c, hipHostMalloc((void**)&c->h_stage[b], doubles * sizeof(double), hipHostMallocDefault),
+                    "hipHostMalloc"))
+            return 0;
+    c->stage_capacity = doubles;
+    return 1;
+}
+
+// Sink for snapshots: host array, file, or nothing.
+struct Sink {
+    sodecl_ctx* c;
+    std::ofstream file;
+    double* dst = nullptr;   // ARRAY destination
+    bool open(size_t total)
+    {
+        if (c->output_type == SODECL_OUTPUT_FILE) {
+            // sodeclmgr.cpp:926-927: appended raw doubles
+            file.open("sodecloutput.bin", std::ios::binary | std::ios::app | std::ios::out);
+            return (bool)file;
+        }
+        if (c->output_type == SODECL_OUTPUT_ARRAY) {
+            if (c->ext_output) {
+                if (c->ext_used + total > c->ext_capacity) return false;
+                dst = c->ext_output + c->ext_used;
+                c->ext_used += total;
+            } else {
+                size_t old = c->output.size();
+                c->output.resize(old + total);
+                dst = c->output.data() + old;
+            }
+        }
+        return true;
+    }
+    void put(const double* host, size_t n)
+    {
+        if (c->output_type == SODECL_OUTPUT_FILE) {
+            file.write((const char*)host, (std::streamsize)(n * sizeof(double)));
+        } else if (dst) {
+            if (host != dst) std::memcpy(dst, host, n * sizeof(double));
+            dst += n;
+        }
+    }
+};
+
+// The reference's launch design (sodeclmgr.cpp:951-1015): blocking read-back
+// of the AoS state before each one-segment launch, host transposition into
+// [equation][orbit] order.
+int run_plain(sodecl_ctx* c, long launches)
+{
+    LaunchCfg cfg{1, 1, c->local_group_size > 0 ? std::min(c->local_group_size, 1024) : 128};
+    if (c->opts.block_size > 0) cfg.block = std::min(c->opts.block_size, 1024);
+    const size_t slab = (size_t)c->neq * c->orbits;
+    Sink sink{c};
+    if (!sink.open(slab * (size_t)(launches + 1))) return fail(c, "output buffer too small");
+    std::vector<double> aos(slab), snap(slab);
+    auto take = [&]() -> bool {
+        if (!hip_ok(c, hipMemcpyAsync(aos.data(), c->d_y, slab * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                    "hipMemcpy(state)"))
+            return false;
+        if (!hip_ok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return false;
+        if (c->output_type == SODECL_OUTPUT_NONE) return true;
+        for (int e = 0; e < c->neq; ++e)
+            for (int o = 0; o < c->orbits; ++o) snap[(size_t)e * c->orbits + o] = aos[(size_t)o * c->neq + e];
+        sink.put(snap.data(), slab);
+        return true;
+    };
+    hipEventRecord(c->ev_start, c->stream);
+    for (long j = 0; j < launches; ++j) {
+        if (!take()) return 0;
+        if (!enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, nullptr, c->d_newton, 0, 1, 0, c->stream))
+            return 0;
+    }
+    if (!take()) return 0;
+    hipEventRecord(c->ev_stop, c->stream);
+    hipEventSynchronize(c->ev_stop);
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    c->last_kernel_ms = ms;
+    c->used_schedule = SODECL_SCHEDULE_PLAIN;
+    return 1;
+}
+
+size_t traj_budget(const sodecl_ctx* c)
+{
+    return c->opts.traj_budget_bytes ? c->opts.traj_budget_bytes : ((size_t)8 << 30);
+}
+
+// Fast path: many segments per launch, snapshots written by the kernel.
+int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
+{
+    const size_t slab = (size_t)c->neq * c->orbits;
+    const size_t nsnap = (size_t)launches + 1;
+    const bool want = c->output_type != SODECL_OUTPUT_NONE;
+    Sink sink{c};
+    if (!sink.open(want ? slab * nsnap : 0)) return fail(c, "output buffer too small");
+    if (!want) {
+        // no output: integrate in as few launches as the options allow
+        long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
+        hipEventRecord(c->ev_start, c->stream);
+        for (long s = 0; s < launches; s += per) {
+            int n = (int)std::min(per, launches - s);
+            if (!enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, nullptr, c->d_newton, 0, n, 0,
+                         c->stream))
+                return 0;
+        }
+        hipEventRecord(c->ev_stop, c->stream);
+        if (!hip_ok(c, hipEventSynchronize(c->ev_stop), "kernel")) return 0;
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+        c->last_kernel_ms = ms;
+        return 1;
+    }
+    // segments per chunk within the buffer budget (two buffers when chunked)
+    long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
+    size_t budget_rows = std::max<size_t>(2, traj_budget(c) / (slab * sizeof(double)));
+    if ((size_t)per + 1 > budget_rows) per = (long)(budget_rows / 2) - 1;
+    if (per < 1) per = 1;
+    const bool single = per >= launches;
+    const size_t rows = (size_t)std::min<long>(per, launches) + 1;
+    if (!ensure_traj(c, rows * slab, single ? 1 : 2)) return 0;
+    const bool to_array_direct = c->output_type == SODECL_OUTPUT_ARRAY && single;
+    if (!to_array_direct && !ensure_stage(c, rows * slab)) return 0;
+
+    hipEventRecord(c->ev_start, c->stream);
+    if (single) {
+        if (!enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, c->d_traj[0], c->d_newton, 0,
+                     (int)launches, 1, c->stream))
+            return 0;
+        hipEventRecord(c->ev_stop, c->stream);
+        double* host = to_array_direct ? sink.dst : c->h_stage[0];
+        if (!hip_ok(c, hipMemcpyAsync(host, c->d_traj[0], nsnap * slab * sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream),
+                    "hipMemcpy(trajectory)"))
+            return 0;
+        if (!hip_ok(c, hipStreamSynchronize(c->stream), "kernel")) return 0;
+        if (to_array_direct)
+            sink.dst += nsnap * slab;
+        else
+            sink.put(host, nsnap * slab);
+    } else {
+        // Chunk k runs on the compute stream into buffer k%2 while the copy
+        // stream moves chunk k-1 to pinned host memory and the host hands
+        // chunk k-2 to the sink.
+        hipEvent_t kdone[2], cdone[2];
+        for (int q = 0; q < 2; ++q) {
+            hipEventCreateWithFlags(&kdone[q], hipEventDisableTiming);
+            hipEventCreateWithFlags(&cdone[q], hipEventDisableTiming);
+        }
+        size_t rows_of[2] = {0, 0};
+        bool ok = true;
+        long k = 0;
+        for (long s = 0; ok && s < launches; s += per, ++k) {
+            const int b = (int)(k & 1);
+            const int n = (int)std::min(per, launches - s);
+            const bool last = s + n >= launches;
+            if (k >= 2) hipStreamWaitEvent(c->stream, cdone[b], 0);
+            ok = enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, c->d_traj[b], c->d_newton, 0, n,
+                         last ? 1 : 0, c->stream);
+            if (!ok) break;
+            hipEventRecord(kdone[b], c->stream);
+            if (k >= 2) {
+                ok = hip_ok(c, hipEventSynchronize(cdone[b]), "copy");
+                if (!ok) break;
+                sink.put(c->h_stage[b], rows_of[b] * slab);
+            }
+            hipStreamWaitEvent(c->copy_stream, kdone[b], 0);
+            rows_of[b] = (size_t)n + (last ? 1 : 0);
+            ok = hip_ok(c, hipMemcpyAsync(c->h_stage[b], c->d_traj[b], rows_of[b] * slab * sizeof(double),
+                                          hipMemcpyDeviceToHost, c->copy_stream),
+                        "hipMemcpy(trajectory)");
+            if (!ok) break;
+            hipEventRecord(cdone[b], c->copy_stream);
+        }
+        hipEventRecord(c->ev_stop, c->stream);
+        // the last two chunks, oldest first
+        for (long q = std::max(0L, k - 2); ok && q < k; ++q) {
+            const int b = (int)(q & 1);
+            ok = hip_ok(c, hipEventSynchronize(cdone[b]), "copy");
+            if (ok) sink.put(c->h_stage[b], rows_of[b] * slab);
+        }
+        if (ok) ok = hip_ok(c, hipStreamSynchronize(c->stream), "kernel");
+        for (int q = 0; q < 2; ++q) {
+            hipEventDestroy(kdone[q]);
+            hipEventDestroy(cdone[q]);
+        }
+        if (!ok) return 0;
+    }
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    c->last_kernel_ms = ms;
+    return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sodecl_default_options(sodecl_options* o)
+{
+    std::memset(o, 0, sizeof *o);
+    o->schedule = SODECL_SCHEDULE_AUTO;
+}
+
+long sodecl_num_launches(double tspan, double dt, int ksteps) { return num_launches(tspan, dt, ksteps); }
+
+int sodecl_create(sodecl_ctx** out, const char* kernel_path, const char* system_path, int solver, double dt,
+                  double tspan, int ksteps, int neq, int npar, int nnoi, int orbits, int output_type)
+{
+    if (!out) return 0;
+    auto* c = new sodecl_ctx();
+    *out = c;
+    c->kernel_path = kernel_path ? kernel_path : "";
+    c->system_path = system_path ? system_path : "";
+    if (!init_ctx(c, solver, dt, tspan, ksteps, neq, npar, nnoi, orbits, output_type)) return 0;
+    if (!read_text(c->system_path, c->system_source))
+        return fail(c, "Error opening file " + c->system_path);   // sodeclmgr.cpp:325-328
+    c->features = detect_features(c->system_source);
+    return 1;
+}
+
+int sodecl_create_from_source(sodecl_ctx** out, const char* system_source, int solver, double dt, double tspan,
+                              int ksteps, int neq, int npar, int nnoi, int orbits, int output_type)
+{
+    if (!out) return 0;
+    auto* c = new sodecl_ctx();
+    *out = c;
+    if (!init_ctx(c, solver, dt, tspan, ksteps, neq, npar, nnoi, orbits, output_type)) return 0;
+    c->system_source = system_source ? system_source : "";
+    c->features = detect_features(c->system_source);
+    return 1;
+}
+
+void sodecl_destroy(sodecl_ctx* c)
+{
+    if (!c) return;
+    if (c->ready || c->stream) {
+        hipSetDevice(c->device);
+        if (c->stream) hipStreamSynchronize(c->stream);
+    }
+    free_device(c);
+    delete c;
+}
+
+int sodecl_choose_device(sodecl_ctx* c, uint32_t platform, uint64_t device_type, uint32_t device)
+{
+    if (!c) return 0;
+    // sodeclmgr.cpp:164-212: out-of-range platform/device or wrong type -> 0
+    if (platform != 0) return fail(c, "Selected platform number is out of bounds.");
+    if (device_type != SODECL_DEVICE_TYPE_ALL && device_type != SODECL_DEVICE_TYPE_GPU)
+        return fail(c, "Selected device is not of the type selected.");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || (int)device >= n)
+        return fail(c, "Selected device number is out of bounds.");
+    c->device = (int)device;
+    c->device_chosen = true;
+    return 1;
+}
+
+void sodecl_set_local_group_size(sodecl_ctx* c, int local_group_size)
+{
+    if (c) c->local_group_size = local_group_size;
+}
+
+void sodecl_set_t0(sodecl_ctx* c, const double* t0)
+{
+    if (c) c->h_t0 = t0;
+}
+void sodecl_set_y0(sodecl_ctx* c, const double* y0)
+{
+    if (c) c->h_y0 = y0;
+}
+void sodecl_set_params(sodecl_ctx* c, const double* params)
+{
+    if (c) c->h_params = params;
+}
+
+int sodecl_set_rng_counters(sodecl_ctx* c, const double* counters)
+{
+    if (!c || !counters) return 0;
+    c->counters.assign(counters, counters + c->orbits);
+    if (c->ready && c->d_counters)
+        return hip_ok(c, hipMemcpy(c->d_counters, c->counters.data(), c->orbits * sizeof(double),
+                                   hipMemcpyHostToDevice),
+                      "hipMemcpy(counters)")
+                   ? 1
+                   : 0;
+    return 1;
+}
+
+int sodecl_get_rng_counters(sodecl_ctx* c, double* counters)
+{
+    if (!c || !counters || c->nnoi <= 0) return 0;
+    if (c->ready && c->d_counters) {
+        if (!hip_ok(c, hipMemcpy(c->counters.data(), c->d_counters, c->orbits * sizeof(double),
+                                 hipMemcpyDeviceToHost),
+                    "hipMemcpy(counters)"))
+            return 0;
+    }
+    std::memcpy(counters, c->counters.data(), c->orbits * sizeof(double));
+    return 1;
+}
+
+int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
+{
+    if (!c || !o) return 0;
+    if (o->schedule < 0 || o->schedule > 3) return fail(c, "invalid schedule");
+    if (o->group_size < 0 || o->group_size > 64) return fail(c, "group_size must be in 0..64");
+    if (o->block_size < 0 || o->block_size > 1024) return fail(c, "block_size must be in 0..1024");
+    bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract;
+    c->opts = *o;
+    if (recompile && c->ready) {
+        for (auto& kv : c->programs)
+            if (kv.second->module) hipModuleUnload(kv.second->module);
+        c->programs.clear();
+    }
+    return 1;
+}
+
+int sodecl_setup(sodecl_ctx* c)
+{
+    if (!c) return 0;
+    if (!c->last_error.empty() && c->system_source.empty()) return 0;
+    if (!c->h_y0 || !c->h_params) return fail(c, "initial state and parameters must be set before setup");
+    if (c->solver == SODECL_STOCHASTIC_EULER && (c->nnoi <= 0 || !c->features.sode_system_stoch))
+        return fail(c, "the stochastic solver needs _numnoi_ > 0 and sode_system_stoch in the system source");
+    if (!c->features.sode_system) return fail(c, "the system source defines no sode_system");
+    if ((c->solver == SODECL_IMPLICIT_EULER || c->solver == SODECL_IMPLICIT_MIDPOINT) && !c->features.calc_jacobian &&
+        !c->features.jacobian_band)
+        return fail(c, "implicit solvers need calc_jacobian (or calc_jacobian_band) in the system source");
+    free_device(c);
+    if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
+    if (!hip_ok(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) return 0;
+    if (!hip_ok(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate")) return 0;
+    hipEventCreate(&c->ev_start);
+    hipEventCreate(&c->ev_stop);
+    const size_t n = (size_t)c->orbits;
+    if (!hip_ok(c, hipMalloc(&c->d_t, n * sizeof(double)), "hipMalloc(t0)")) return 0;
+    if (!hip_ok(c, hipMalloc(&c->d_y, n * c->neq * sizeof(double)), "hipMalloc(y0)")) return 0;
+    if (!hip_ok(c, hipMalloc(&c->d_params, std::max<size_t>(1, n * c->npar) * sizeof(double)), "hipMalloc(params)"))
+        return 0;
+    if (!hip_ok(c, hipMalloc(&c->d_newton, n * sizeof(long long)), "hipMalloc(newton)")) return 0;
+    hipMemsetAsync(c->d_newton, 0, n * sizeof(long long), c->stream);
+    std::vector<double> zeros;
+    const double* t0 = c->h_t0;
+    if (!t0) {
+        zeros.assign(n, 0.0);
+        t0 = zeros.data();
+    }
+    if (!hip_ok(c, hipMemcpy(c->d_t, t0, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(t0)")) return 0;
+    if (!hip_ok(c, hipMemcpy(c->d_y, c->h_y0, n * c->neq * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(y0)"))
+        return 0;
+    if (c->npar > 0 &&
+        !hip_ok(c, hipMemcpy(c->d_params, c->h_params, n * c->npar * sizeof(double), hipMemcpyHostToDevice),
+                "hipMemcpy(params)"))
+        return 0;
+    if (c->nnoi > 0) {
+        if (!hip_ok(c, hipMalloc(&c->d_counters, n * sizeof(double)), "hipMalloc(counters)")) return 0;
+        if (!hip_ok(c, hipMemcpy(c->d_counters, c->counters.data(), n * sizeof(double), hipMemcpyHostToDevice),
+                    "hipMemcpy(counters)"))
+            return 0;
+    }
+    // compile the program the first run will use (JIT outside the timed loop)
+    LaunchCfg cfg = choose_config(c);
+    if (!get_program(c, cfg.schedule, cfg.group, cfg.block)) return 0;
+    hipStreamSynchronize(c->stream);
+    c->ready = true;
+    return 1;
+}
+
+int sodecl_run(sodecl_ctx* c)
+{
+    if (!c) return 0;
+    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
+    if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
+    const long launches = num_launches(c->tspan, c->dt, c->ksteps);
+    if (launches < 0) return fail(c, "too many kernel launches");
+    auto t0 = std::chrono::steady_clock::now();
+    int ok;
+    if (c->opts.schedule == SODECL_SCHEDULE_PLAIN)
+        ok = run_plain(c, launches);
+    else
+        ok = run_fast(c, launches, choose_config(c));
+    auto t1 = std::chrono::steady_clock::now();
+    c->last_runtime = std::chrono::duration<double>(t1 - t0).count();
+    if (ok && c->opts.verbose)
+        std::cout << "Compute results runtime: " << c->last_runtime << " sec.\n";   // sodeclmgr.cpp:1019-1020
+    return ok;
+}
+
+int sodecl_run_device(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,
+                      void* stream)
+{
+    if (!c) return 0;
+    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
+    if (!t0 || !y0 || (c->npar > 0 && !params)) return fail(c, "null device pointer");
+    if (c->solver == SODECL_STOCHASTIC_EULER && !counters) return fail(c, "the stochastic solver needs counters");
+    if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
+    const long launches = num_launches(c->tspan, c->dt, c->ksteps);
+    if (launches < 0) return fail(c, "too many kernel launches");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    LaunchCfg cfg = choose_config(c);
+    long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
+    if (launches == 0) {
+        if (traj)
+            return enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, 0, 0, 1, s);
+        return 1;
+    }
+    for (long s0 = 0; s0 < launches; s0 += per) {
+        int n = (int)std::min(per, launches - s0);
+        bool last = s0 + n >= launches;
+        if (!enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, (int)s0, n, last ? 1 : 0, s)) return 0;
+    }
+    return 1;
+}
+
+size_t sodecl_output_size(const sodecl_ctx* c)
+{
+    if (!c) return 0;
+    return c->ext_output ? c->ext_used : c->output.size();
+}
+
+const double* sodecl_output(const sodecl_ctx* c)
+{
+    if (!c) return nullptr;
+    return c->ext_output ? c->ext_output : c->output.data();
+}
+
+int sodecl_set_output_buffer(sodecl_ctx* c, double* host, size_t capacity)
+{
+    if (!c) return 0;
+    c->ext_output = host;
+    c->ext_capacity = host ? capacity : 0;
+    c->ext_used = 0;
+    return 1;
+}
+
+double sodecl_last_runtime(const sodecl_ctx* c) { return c ? c->last_runtime : 0.0; }
+double sodecl_last_kernel_ms(const sodecl_ctx* c) { return c ? c->last_kernel_ms : 0.0; }
+
+int sodecl_get_launch_config(const sodecl_ctx* c, int* schedule, int* group, int* block)
+{
+    if (!c) return 0;
+    if (schedule) *schedule = c->used_schedule;
+    if (group) *group = c->used_group;
+    if (block) *block = c->used_block;
+    return 1;
+}
+
+int sodecl_get_newton_updates(sodecl_ctx* c, long long* counts)
+{
+    if (!c || !counts || !c->ready) return 0;
+    return hip_ok(c, hipMemcpy(counts, c->d_newton, c->orbits * sizeof(long long), hipMemcpyDeviceToHost),
+                  "hipMemcpy(newton)")
+               ? 1
+               : 0;
+}
+
+size_t sodecl_compile_only(sodecl_ctx* c, const char* arch)
+{
+    if (!c) return 0;
+    LaunchCfg cfg = choose_config(c);
+    ProgramKey key;
+    key.solver = c->solver;
+    key.neq = c->neq;
+    key.npar = c->npar;
+    key.nnoi = c->nnoi;
+    key.orbits = c->orbits;
+    key.ksteps = c->ksteps;
+    key.dt = c->dt;
+    key.schedule = cfg.schedule;
+    key.group = cfg.group;
+    key.block = cfg.block;
+    key.newton_max_iter = c->opts.newton_max_iter > 0 ? c->opts.newton_max_iter : 1000;
+    key.fp_contract = c->opts.fp_contract;
+    c->last_program = assemble_program(c->system_source, key, c->features);
+    std::vector<char> code;
+    std::string log;
+    if (!compile_program(c->last_program, arch ? arch : "gfx950", key.fp_contract, code, log)) {
+        fail(c, "Failed to build program executable:\n" + log);
+        return 0;
+    }
+    return code.size();
+}
+
+const char* sodecl_program_source(sodecl_ctx* c) { return c ? c->last_program.c_str() : ""; }
+
+const char* sodecl_last_error(const sodecl_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+int sodecl_read_binary_data_from_file(const char* filename, double* data, size_t capacity, size_t* count)
+{
+    // sodeclmgr.cpp:839-867: the whole file as raw doubles
+    std::ifstream f(filename, std::ios::binary | std::ios::ate);
+    if (!f) {
+        std::cout << "ERROR: can't open " << filename << " file." << std::endl;
+        return 0;
+    }
+    std::streamsize size = f.tellg();
+    f.seekg(0, std::ios::beg);
+    size_t n = (size_t)size / sizeof(double);
+    if (n > capacity) n = capacity;
+    f.read((char*)data, (std::streamsize)(n * sizeof(double)));
+    if (count) *count = n;
+    return 1;
+}
+
+int sodecl_read_data_from_file(const char* filename, double* data, size_t capacity, size_t* count)
+{
+    // sodeclmgr.cpp:877-914: whitespace-separated numbers, line by line
+    std::ifstream f(filename);
+    if (!f) {
+        std::cout << "ERROR: can't open " << filename << " file." << std::endl;
+        return 0;
+    }
+    size_t j = 0;
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream iss(line);
+        std::string tok;
+        while (iss >> tok) {
+            if (j < capacity) data[j] = std::atof(tok.c_str());
+            ++j;
+        }
+    }
+    if (count) *count = std::min(j, capacity);
+    return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal declarations shared by the translation units of libsodecl_amd.so.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sodecl.h"
+
+namespace sodecl_amd {
+
+// Optional plugin entry points found in a system source.
+struct PluginFeatures {
+    bool sode_system = false;
+    bool sode_system_stoch = false;
+    bool calc_jacobian = false;
+    bool component = false;         // double sode_system_i(int, t, y[], p[])
+    bool stoch_component = false;   // double sode_system_stoch_i(int, t, y[], p[], noise[])
+    bool jacobian_band = false;     // calc_jacobian_band(y, lower, diag, upper, p)
+    bool jacobian_band_i = false;   // calc_jacobian_band_i(i, y, p, *lower, *diag, *upper)
+    bool vjp_group = false;         // sode_system_vjp_group(...) (adjoint, group schedule)
+};
+
+PluginFeatures detect_features(const std::string& source);
+std::string strip_comments(const std::string& source);
+
+// Everything that selects one compiled program.
+struct ProgramKey {
+    int solver = 2;
+    int neq = 1, npar = 0, nnoi = 0, orbits = 1, ksteps = 1;
+    double dt = 0.0;
+    int schedule = 1;    // 1 lane, 2 group
+    int group = 64;      // lanes per orbit (group schedule)
+    int block = 128;
+    int newton_max_iter = 1000;
+    int fp_contract = 0;
+    std::string extra_defines;   // additional "#define" lines (adjoint, batch)
+};
+
+// printf("%g") of dt, as the reference's double2string (sodeclmgr.cpp:356-367).
+std::string dt_literal(double dt);
+
+// Program text handed to hipRTC: macro prelude, OpenCL-C compatibility,
+// the user system source, the integrator templates.
+std::string assemble_program(const std::string& system_source, const ProgramKey& key, const PluginFeatures& f);
+
+// A compiled and loaded program.
+struct Program {
+    std::vector<char> code;
+    hipModule_t module = nullptr;
+    int device = -1;
+    std::map<std::string, hipFunction_t> functions;
+    hipFunction_t get(const std::string& name);   // null when absent
+};
+
+// Compiles (through an in-memory and on-disk cache keyed by the program
+// text, the options and the architecture) and returns the code object.
+bool compile_program(const std::string& text, const std::string& arch, int fp_contract, std::vector<char>& code,
+                     std::string& log);
+std::string device_arch(int device);
+
+long num_launches(double tspan, double dt, int ksteps);
+
+}  // namespace sodecl_amd
+
+struct sodecl_ctx {
+    // configuration (sodeclmgr ctor arguments, sodeclmgr.cpp:26-83)
+    std::string kernel_path;
+    std::string system_path;
+    std::string system_source;
+    int solver = 2;
+    double dt = 0.0, tspan = 0.0;
+    int ksteps = 1, neq = 1, npar = 0, nnoi = 0, orbits = 1;
+    int output_type = SODECL_OUTPUT_ARRAY;
+    int local_group_size = 0;
+    int device = 0;
+    bool device_chosen = false;
+    sodecl_options opts{};
+    sodecl_amd::PluginFeatures features;
+
+    // stashed host inputs (read at setup)
+    const double* h_t0 = nullptr;
+    const double* h_y0 = nullptr;
+    const double* h_params = nullptr;
+    std::vector<double> counters;   // per-orbit RNG counters (SDE)
+
+    // device state
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;
+    double* d_t = nullptr;
+    double* d_y = nullptr;
+    double* d_params = nullptr;
+    double* d_counters = nullptr;
+    long long* d_newton = nullptr;
+    double* d_traj[2] = {nullptr, nullptr};
+    size_t traj_capacity = 0;       // doubles per trajectory buffer
+    double* h_stage[2] = {nullptr, nullptr};   // pinned staging for host output
+    size_t stage_capacity = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+
+    // programs: one per (schedule, group, block)
+    std::map<std::string, std::shared_ptr<sodecl_amd::Program>> programs;
+    int used_schedule = 0, used_group = 0, used_block = 0;
+    bool tuned = false;
+
+    // output (sodeclmgr::m_output)
+    std::vector<double> output;
+    double* ext_output = nullptr;
+    size_t ext_capacity = 0;
+    size_t ext_used = 0;
+
+    double last_runtime = 0.0;
+    double last_kernel_ms = 0.0;
+    std::string last_error;
+    std::string last_program;
+
+    // hipGraph of the launch sequence of a run
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::string graph_key;
+};
+
+namespace sodecl_amd {
+int fail(sodecl_ctx* ctx, const std::string& msg);
+bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what);
+// Program for the given schedule on the context's device (compiled on first use).
+std::shared_ptr<Program> get_program(sodecl_ctx* ctx, int schedule, int group, int block,
+                                     const std::string& extra_defines = std::string());
+}  // namespace sodecl_amd
