@@ -17,6 +17,14 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 SYSTEMS_DIR = os.path.join(HERE, "systems")
 
+# One HIP runtime per process: PyTorch ships its own ROCm runtime libraries
+# and loads them by path.  Importing it first lets libsodecl_amd.so bind to
+# that same runtime (same sonames) instead of loading a second one.
+try:  # pragma: no cover - depends on the environment
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 try:
     from . import sodecl_interface  # noqa: F401
 except ImportError as e:  # pragma: no cover - exercised only without a build

@@ -30,24 +30,24 @@
 #define OCU_TN 25.00
 
 /* Burst-neuron firing curve: saturating exponential for x > 0, a scaled
- * x*exp(x/b) ramp otherwise. */
+ * x*exp(x/b) ramp otherwise.  The exponent is selected before the single
+ * exp() call, so lanes of a wavefront that take different sides still
+ * evaluate one exponential; each side computes exactly the operations of
+ * the two-branch form ((-x)/bt or x/b, then exp); numerator and denominator
+ * are selected first so the division is evaluated once as well. */
 double ocu_burst(double x, double a, double b, double at, double bt)
 {
-    if (x > 0)
-    {
-        return at * (1.00 - exp(-x / bt));
-    }
-    return -(a / b) * x * exp(x / b);
+    const int pos = x > 0;
+    const double e = exp((pos ? -x : x) / (pos ? bt : b));
+    return pos ? at * (1.00 - e) : -(a / b) * x * e;
 }
 
 /* d(ocu_burst)/dx */
 double ocu_burst_dx(double x, double a, double b, double at, double bt)
 {
-    if (x > 0)
-    {
-        return (at * exp(-x / bt)) / bt;
-    }
-    return -((a * exp(x / b)) / b + (a * x * exp(x / b)) / (b * b));
+    const int pos = x > 0;
+    const double e = exp((pos ? -x : x) / (pos ? bt : b));
+    return pos ? (at * e) / bt : -((a * e) / b + (a * x * e) / (b * b));
 }
 
 void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])

@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# PyTorch loads its own ROCm runtime by path; import it before the native
+# library so the process holds one HIP runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
