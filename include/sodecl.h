@@ -153,6 +153,27 @@ long sodecl_num_launches(double tspan, double dt, int ksteps);
 int sodecl_run_device(sodecl_ctx* ctx, double* t0, double* y0, const double* params, double* counters,
                       double* traj, void* stream);
 
+/* ---- Discrete adjoint (no reference counterpart) ----------------------
+ * For the Euler and RK4 solvers under the group schedule, with a plugin
+ * that defines sode_system_i and sode_system_vjp_group.  Given dL/dtraj for
+ * a trajectory of the context's configuration, returns dL/dy0[orbits][neq]
+ * and dL/dparams[orbits][npar], the exact transpose of the forward steps.
+ * Memory is bounded by checkpointing every ckpt_every steps
+ * (sodecl_default_checkpoint_interval: ceil(sqrt(steps))). */
+int sodecl_default_checkpoint_interval(const sodecl_ctx* ctx);
+/* Number of checkpoints: ceil(steps / ckpt_every); ckpt_y holds that many
+ * [neq][orbits] states, ckpt_t that many [orbits] times. */
+long sodecl_checkpoint_count(const sodecl_ctx* ctx, int ckpt_every);
+/* Doubles of device workspace sodecl_adjoint_device needs. */
+size_t sodecl_adjoint_workspace_size(const sodecl_ctx* ctx, int ckpt_every);
+/* sodecl_run_device that also writes the checkpoints (device memory). */
+int sodecl_run_device_checkpointed(sodecl_ctx* ctx, double* t0, double* y0, const double* params, double* counters,
+                                   double* traj, double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream);
+/* Backward sweep: cot is dL/dtraj [nsnap][neq][orbits]; all device memory. */
+int sodecl_adjoint_device(sodecl_ctx* ctx, const double* ckpt_t, const double* ckpt_y, const double* params,
+                          const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
+                          void* stream);
+
 /* Wall time of the last sodecl_run segment loop in seconds, and the kernel
  * time of the last run measured with HIP events on the launch stream. */
 double sodecl_last_runtime(const sodecl_ctx* ctx);

@@ -72,6 +72,9 @@ class Oracle:
         L.orc_num_launches.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_rhs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         L.orc_noise_pair.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, _dp]
+        L.orc_sensitivity_gradient.restype = ctypes.c_long
+        L.orc_sensitivity_gradient.argtypes = [ctypes.POINTER(_Config), ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                               _dp, _dp]
         L.orc_counter_word.restype = ctypes.c_uint64
         L.orc_counter_word.argtypes = [ctypes.c_double]
 
@@ -96,6 +99,25 @@ class Oracle:
         if r < 0:
             raise ValueError("oracle rejected the configuration")
         return dict(out=out, y=y, t=t, counters=cnt, newton_updates=nu, launches=int(r))
+
+    def sensitivity_gradient(self, solver, y0, params, cot, dt, tspan, ksteps, orbit_range=None, nthreads=0):
+        """Gradient of sum <cot[snap], y(snap)> w.r.t. y0 and params by forward
+        sensitivity; cot is [nsnap][neq][orbits].  Returns (grad_y0, grad_p)
+        for the orbit range."""
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        cot = np.ascontiguousarray(cot, dtype=np.float64)
+        orbits, neq = y0.shape
+        npar = params.shape[1]
+        c = _Config(SOLVERS[solver], neq, npar, 0, orbits, ksteps, dt, tspan, 1000, nthreads)
+        ob, oc = (0, orbits) if orbit_range is None else orbit_range
+        gy = np.zeros((oc, neq))
+        gp = np.zeros((oc, npar))
+        r = self.lib.orc_sensitivity_gradient(ctypes.byref(c), ob, oc, _ptr(y0), _ptr(params), _ptr(cot), _ptr(gy),
+                                              _ptr(gp))
+        if r < 0:
+            raise ValueError("sensitivity needs calc_jacobian/calc_jacobian_params and an explicit solver")
+        return gy, gp
 
     def threefry(self, ctr, key):
         c = (ctypes.c_uint64 * 2)(*ctr)

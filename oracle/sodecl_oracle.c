@@ -67,6 +67,7 @@ int orc_newton_maxit = 1000;   /* ie.cl:69, im.cl:65 */
 /* Optional plugin entry points: absent ones resolve to NULL. */
 void sode_system_stoch(double t, double* y, double* stoch, double* p, double* noise) __attribute__((weak));
 void calc_jacobian(double* y, double (*jac)[orc_numeq], double* p) __attribute__((weak));
+void calc_jacobian_params(double* y, double (*jp)[orc_numpar], double* p) __attribute__((weak));
 
 enum { ORC_SEULER = 0, ORC_EULER = 1, ORC_RK4 = 2, ORC_IE = 3, ORC_IM = 4 };
 
@@ -445,4 +446,114 @@ void orc_noise_pair(uint64_t orbit, uint64_t counter_word, double dt, double out
     double u1 = orc_u01_open_open_64_53(rr[1]);
     out[0] = (sqrt(-2 * log(u0)) * cos(2 * M_PI * u1)) * dts;
     out[1] = (sqrt(-2 * log(u0)) * sin(2 * M_PI * u1)) * dts;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Forward sensitivity (tangent-linear) of the explicit solvers: the FP64    */
+/* reference for the MI355X discrete adjoint.  No reference counterpart: the */
+/* reference computes no derivatives.                                        */
+/* ------------------------------------------------------------------------ */
+
+/* dk = J(x) dx + [0 | df/dp(x)], all (neq x ncol) with ncol = neq + npar */
+static void orc_tangent_rhs(double t, double* x, double* p, const double* dx, double* dk, int ncol)
+{
+    int n = _numeq_, np = _numpar_;
+    double (*jac)[n] = malloc(sizeof(double) * (size_t)n * (size_t)n);
+    double (*jp)[np > 0 ? np : 1] = malloc(sizeof(double) * (size_t)n * (size_t)(np > 0 ? np : 1));
+    (void)t;
+    calc_jacobian(x, jac, p);
+    if (np > 0) calc_jacobian_params(x, jp, p);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < ncol; k++) {
+            double acc = 0;
+            for (int m = 0; m < n; m++) acc += jac[i][m] * dx[(size_t)m * ncol + k];
+            if (k >= n) acc += jp[i][k - n];
+            dk[(size_t)i * ncol + k] = acc;
+        }
+    }
+    free(jac);
+    free(jp);
+}
+
+/*
+ * Gradient of L = sum_snap <cot[snap], y(snap)> with respect to y0 and the
+ * parameters of each orbit in [orbit_begin, orbit_begin + orbit_count), by
+ * propagating S = d y / d(y0, p) through the same discrete steps as the
+ * forward integration (Euler or RK4).  cot is [nsnap][neq][orbits] over all
+ * orbits; grad_y0 [count][neq], grad_p [count][npar].  Returns 0, or -1 when
+ * the plugin lacks calc_jacobian / calc_jacobian_params or the solver is
+ * not explicit.
+ */
+long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_count, const double* y0,
+                              const double* params, const double* cot, double* grad_y0, double* grad_p)
+{
+    if (!calc_jacobian || (c->npar > 0 && !calc_jacobian_params)) return -1;
+    if (c->solver != ORC_EULER && c->solver != ORC_RK4) return -1;
+    orc_set_globals(c);
+    long launches = orc_num_launches(c->tspan, c->dt, c->ksteps);
+    int n = c->neq, np = c->npar, ncol = n + np;
+    double dt = _m_dt_;
+#ifdef _OPENMP
+    int nth = c->nthreads > 0 ? c->nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
+#endif
+    for (int o = 0; o < orbit_count; o++) {
+        int orbit = orbit_begin + o;
+        double* S = calloc((size_t)n * ncol, sizeof(double));
+        double* dk = calloc((size_t)n * ncol, sizeof(double));
+        double* acc = calloc((size_t)n * ncol, sizeof(double));
+        double* dxs = calloc((size_t)n * ncol, sizeof(double));
+        double* g = calloc((size_t)ncol, sizeof(double));
+        double y[n], pp[np > 0 ? np : 1], f[n], k[n], xs[n], yo[n];
+        for (int i = 0; i < n; i++) {
+            y[i] = y0[(long)orbit * n + i];
+            S[(size_t)i * ncol + i] = 1.0;
+        }
+        for (int m = 0; m < np; m++) pp[m] = params[(long)orbit * np + m];
+        double t = 0;
+        long total = launches * c->ksteps;
+        for (long s = 0; s <= total; s++) {
+            if (s % c->ksteps == 0) {
+                const double* cs = cot + (s / c->ksteps) * (long)n * c->orbits;
+                for (int col = 0; col < ncol; col++) {
+                    double a = 0;
+                    for (int i = 0; i < n; i++) a += cs[(long)i * c->orbits + orbit] * S[(size_t)i * ncol + col];
+                    g[col] += a;
+                }
+            }
+            if (s == total) break;
+            if (c->solver == ORC_EULER) {
+                orc_tangent_rhs(t, y, pp, S, dk, ncol);
+                sode_system(t, y, f, pp);
+                for (size_t q = 0; q < (size_t)n * ncol; q++) S[q] = S[q] + dk[q] * dt;
+                for (int i = 0; i < n; i++) y[i] = y[i] + f[i] * dt;
+            } else {
+                /* stage 1 */
+                orc_tangent_rhs(t, y, pp, S, dk, ncol);
+                sode_system(t, y, f, pp);
+                for (size_t q = 0; q < (size_t)n * ncol; q++) { acc[q] = dk[q]; dxs[q] = S[q] + (dt * 0.5) * dk[q]; }
+                for (int i = 0; i < n; i++) { yo[i] = f[i]; xs[i] = y[i] + (dt * 0.50) * f[i]; }
+                /* stage 2 */
+                orc_tangent_rhs(t + 0.50 * dt, xs, pp, dxs, dk, ncol);
+                sode_system(t + 0.50 * dt, xs, k, pp);
+                for (size_t q = 0; q < (size_t)n * ncol; q++) { acc[q] += 2 * dk[q]; dxs[q] = S[q] + (dt * 0.5) * dk[q]; }
+                for (int i = 0; i < n; i++) { yo[i] = yo[i] + 2.00 * k[i]; xs[i] = y[i] + (dt * 0.50) * k[i]; }
+                /* stage 3 */
+                orc_tangent_rhs(t + 0.50 * dt, xs, pp, dxs, dk, ncol);
+                sode_system(t + 0.50 * dt, xs, k, pp);
+                for (size_t q = 0; q < (size_t)n * ncol; q++) { acc[q] += 2 * dk[q]; dxs[q] = S[q] + dt * dk[q]; }
+                for (int i = 0; i < n; i++) { yo[i] = yo[i] + 2.00 * k[i]; xs[i] = y[i] + dt * k[i]; }
+                /* stage 4 */
+                orc_tangent_rhs(t + dt, xs, pp, dxs, dk, ncol);
+                sode_system(t + dt, xs, k, pp);
+                for (size_t q = 0; q < (size_t)n * ncol; q++) S[q] = S[q] + (acc[q] + dk[q]) * dt / 6.0;
+                for (int i = 0; i < n; i++) y[i] = y[i] + (yo[i] + k[i]) * dt / 6.00;
+            }
+            t = t + dt;
+        }
+        for (int i = 0; i < n; i++) grad_y0[(long)o * n + i] = g[i];
+        for (int m = 0; m < np; m++) grad_p[(long)o * np + m] = g[n + m];
+        free(S); free(dk); free(acc); free(dxs); free(g);
+    }
+    return 0;
 }

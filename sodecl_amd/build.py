@@ -30,21 +30,26 @@ def run(cmd, **kw):
     subprocess.run(cmd, check=True, **kw)
 
 
-def embed_device_sources():
-    """device/*.hip -> _embedded_device.inc as one C++ raw string."""
+def _raw_string(var, names):
     parts = []
-    for name in DEVICE_SOURCES:
+    for name in names:
         path = os.path.join(CSRC, "device", name)
         if os.path.exists(path):
             parts.append(f'#line 1 "{name}"\n' + open(path).read())
     text = "\n".join(parts)
     delim = "SODECL_EMBED"
     assert f"){delim}\"" not in text
-    out = os.path.join(CSRC, "_embedded_device.inc")
     # keep long strings under compiler limits by concatenating chunks
     chunks = [text[i:i + 60000] for i in range(0, len(text), 60000)]
     body = "\n".join(f'R"{delim}({c}){delim}"' for c in chunks)
-    new = f"static const char* const kIntegratorsSource =\n{body};\n"
+    return f"static const char* const {var} =\n{body};\n"
+
+
+def embed_device_sources():
+    """device/*.hip -> _embedded_device.inc (C++ raw strings): the prelude
+    emitted before the system source and the integrators emitted after it."""
+    out = os.path.join(CSRC, "_embedded_device.inc")
+    new = _raw_string("kPreludeSource", ["prelude.hip"]) + _raw_string("kIntegratorsSource", DEVICE_SOURCES)
     if not os.path.exists(out) or open(out).read() != new:
         with open(out, "w") as f:
             f.write(new)

@@ -1,0 +1,101 @@
+// C ABI of the discrete adjoint (include/sodecl.h, no reference counterpart).
+//
+// Forward: sodecl_run_device_checkpointed stores the state every ckpt_every
+// steps while it integrates.  Backward: sodecl_adjoint_device recomputes each
+// checkpoint segment and sweeps it backwards (device/adjoint.hip), giving
+// dL/dy0 and dL/dparams of every orbit from dL/d(trajectory).
+#include <cmath>
+
+#include "sodecl_internal.hpp"
+
+using namespace sodecl_amd;
+
+namespace {
+
+const char* kAdjointDefines = "#define SODECL_ADJOINT 1\n";
+
+int nstage(const sodecl_ctx* c) { return c->solver == SODECL_RUNGE_KUTTA ? 4 : 1; }
+
+long total_steps(const sodecl_ctx* c) { return num_launches(c->tspan, c->dt, c->ksteps) * (long)c->ksteps; }
+
+int check_adjoint(sodecl_ctx* c, int& sched, int& group, int& block)
+{
+    if (c->solver != SODECL_EULER && c->solver != SODECL_RUNGE_KUTTA)
+        return fail(c, "the adjoint supports the Euler and RK4 solvers");
+    if (!c->features.component || !c->features.vjp_group)
+        return fail(c, "the adjoint needs sode_system_i and sode_system_vjp_group in the system source");
+    launch_config(c, &sched, &group, &block);
+    if (sched != SODECL_SCHEDULE_GROUP)
+        return fail(c, "the adjoint runs under the group schedule (set schedule to group)");
+    return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sodecl_default_checkpoint_interval(const sodecl_ctx* c)
+{
+    if (!c) return 0;
+    long s = total_steps(c);
+    if (s <= 0) return 1;
+    return (int)std::max(1.0, std::ceil(std::sqrt((double)s)));
+}
+
+long sodecl_checkpoint_count(const sodecl_ctx* c, int ckpt_every)
+{
+    if (!c || ckpt_every <= 0) return 0;
+    long s = total_steps(c);
+    return (s + ckpt_every - 1) / ckpt_every;
+}
+
+size_t sodecl_adjoint_workspace_size(const sodecl_ctx* c, int ckpt_every)
+{
+    if (!c || ckpt_every <= 0) return 0;
+    int sched = 0, group = 0, block = 0;
+    launch_config(c, &sched, &group, &block);
+    if (sched != SODECL_SCHEDULE_GROUP) return 0;
+    const int opb = block / group;
+    const size_t padded = (size_t)((c->orbits + opb - 1) / opb) * opb;
+    return padded * (size_t)ckpt_every * (size_t)(nstage(c) * c->neq + 1);
+}
+
+int sodecl_run_device_checkpointed(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters,
+                                   double* traj, double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream)
+{
+    if (!c) return 0;
+    if (!ckpt_t || !ckpt_y || ckpt_every <= 0) return fail(c, "checkpoint buffers and interval required");
+    return run_device_impl(c, t0, y0, params, counters, traj, ckpt_t, ckpt_y, ckpt_every, stream);
+}
+
+int sodecl_adjoint_device(sodecl_ctx* c, const double* ckpt_t, const double* ckpt_y, const double* params,
+                          const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
+                          void* stream)
+{
+    if (!c) return 0;
+    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
+    if (!ckpt_t || !ckpt_y || !cot || !grad_y0 || (c->npar > 0 && (!params || !grad_params)) || !work)
+        return fail(c, "null device pointer");
+    if (ckpt_every <= 0) return fail(c, "ckpt_every must be positive");
+    int sched = 0, group = 0, block = 0;
+    if (!check_adjoint(c, sched, group, block)) return 0;
+    if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
+    auto prog = get_program(c, sched, group, block, kAdjointDefines);
+    if (!prog) return 0;
+    hipFunction_t fn = prog->get("sodecl_group_adjoint");
+    if (!fn) return fail(c, "kernel sodecl_group_adjoint missing from the program");
+    const long steps = total_steps(c);
+    if (steps > 2147483647L) return fail(c, "too many steps");
+    int orbits = c->orbits, nsteps = (int)steps, every = ckpt_every;
+    const int opb = block / group;
+    const unsigned grid = (unsigned)((orbits + opb - 1) / opb);
+    void* args[] = {(void*)&ckpt_t, (void*)&ckpt_y, (void*)&params, (void*)&cot, &grad_y0, &grad_params,
+                    &work, &orbits, &nsteps, &every};
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (!hip_ok(c, hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)block, 1, 1, 0, s, args, nullptr),
+                "hipModuleLaunchKernel(adjoint)"))
+        return 0;
+    return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+// Discrete adjoint of the explicit solvers under the group schedule.
+//
+// Gradients of L(trajectory) with respect to each orbit's initial state and
+// parameters, given dL/d(snapshot) for every snapshot.  The adjoint is the
+// exact transpose of the forward step as the forward kernel computes it
+// (classic RK4 of rk4.cl:17-64 or Euler of euler.cl:10-18), so it matches a
+// forward-sensitivity computation to rounding.
+//
+// Memory is bounded by checkpointing (Griewank's revolve with one level):
+// the forward kernel stores the state every ckpt_every steps (sqrt of the
+// step count by default); here each segment is recomputed from its
+// checkpoint, with the stage vectors of every step written to a per-orbit
+// workspace, and then swept backwards.  The plugin supplies the
+// vector-Jacobian product of its drift, evaluated cooperatively by the group:
+//
+//   void sode_system_vjp_group(double t, double x[], double lam[], double p[],
+//                              double gx[], double gp[])
+//     gx[i] = sum_k lam[k] df_k/dx_i for this lane's components
+//             i = sodecl_lane() + c * sodecl_group_size();
+//     gp[k] += sum_i lam[i] df_i/dp_k, every entry updated by one lane only.
+#if SODECL_SCHEDULE == 2 && defined(SODECL_ADJOINT) && SODECL_HAS_VJP_GROUP && \
+    (SODECL_SOLVER == 1 || SODECL_SOLVER == 2)
+
+// One VJP through LDS: publish point v and cotangent l, barrier, evaluate.
+SODECL_INLINE void sodecl_group_vjp(double t, const double (&v)[SODECL_CPL], const double (&l)[SODECL_CPL],
+                                    double (&g)[SODECL_CPL], double (*sv)[SODECL_OPB][SODECL_XSTRIDE],
+                                    double (*sl)[SODECL_OPB][SODECL_XSTRIDE], double* sg, int& vb, int slot,
+                                    int lane, double* pp, double* gp)
+{
+    sodecl_publish(sv[vb][slot], v, lane);
+    sodecl_publish(sl[vb][slot], l, lane);
+    __syncthreads();
+    sode_system_vjp_group(t, sv[vb][slot], sl[vb][slot], pp, sg, gp);
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        g[c] = comp < _numeq_ ? sg[comp] : 0.0;
+    }
+    vb ^= 1;
+}
+
+// Per-orbit workspace row layout: for each step of a segment, NSTAGE stage
+// vectors of neq doubles, then the step's t.
+#define SODECL_WROW (SODECL_NSTAGE * _numeq_ + 1)
+
+extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
+sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict__ ckpt_y,
+                     const double* __restrict__ params, const double* __restrict__ cot,
+                     double* __restrict__ grad_y0, double* __restrict__ grad_p, double* __restrict__ work,
+                     int orbits, int total_steps, int ckpt_every)
+{
+    __shared__ double sh_x[2][SODECL_OPB][SODECL_XSTRIDE];
+    __shared__ double sh_v[2][SODECL_OPB][SODECL_XSTRIDE];
+    __shared__ double sh_l[2][SODECL_OPB][SODECL_XSTRIDE];
+    __shared__ double sh_g[SODECL_OPB][SODECL_XSTRIDE];
+    __shared__ double sh_p[SODECL_OPB][SODECL_NPAR_ALLOC];
+    __shared__ double sh_gp[SODECL_OPB][SODECL_NPAR_ALLOC];
+    const int lane = threadIdx.x % SODECL_G;
+    const int slot = threadIdx.x / SODECL_G;
+    const int orbit = blockIdx.x * SODECL_OPB + slot;
+    const bool active = orbit < orbits;
+    const int ob = active ? orbit : 0;   // inactive groups read orbit 0's inputs and store no result
+    double* const pp = sh_p[slot];
+    double* const gp = sh_gp[slot];
+    for (int k = lane; k < _numpar_; k += SODECL_G) {
+        pp[k] = params[(size_t)ob * _numpar_ + k];
+        gp[k] = 0.0;
+    }
+    const size_t slab = (size_t)_numeq_ * (size_t)orbits;
+    double lam[SODECL_CPL];
+    // cotangent of the final snapshot (step total_steps)
+    {
+        const long long last = total_steps / _numsteps_;
+#pragma unroll
+        for (int c = 0; c < SODECL_CPL; ++c) {
+            const int comp = lane + c * SODECL_G;
+            lam[c] = comp < _numeq_ ? cot[(size_t)last * slab + (size_t)comp * orbits + ob] : 0.0;
+        }
+    }
+    // the workspace is sized for every group of the grid, active or not
+    double* const wk = work + (size_t)orbit * ckpt_every * SODECL_WROW;
+    const int nseg = (total_steps + ckpt_every - 1) / ckpt_every;
+    int buf = 0, vb = 0;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int sb = seg * ckpt_every;
+        const int se = min(total_steps, sb + ckpt_every);
+        double y[SODECL_CPL];
+#pragma unroll
+        for (int c = 0; c < SODECL_CPL; ++c) {
+            const int comp = lane + c * SODECL_G;
+            y[c] = comp < _numeq_ ? ckpt_y[(size_t)seg * slab + (size_t)comp * orbits + ob] : 0.0;
+        }
+        double t = ckpt_t[(size_t)seg * orbits + ob];
+        // forward recompute of the segment, same arithmetic as the forward kernel
+        for (int s = sb; s < se; ++s) {
+            double* row = wk + (size_t)(s - sb) * SODECL_WROW;
+            if (lane == 0) row[SODECL_NSTAGE * _numeq_] = t;
+            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, row);
+            t = t + _m_dt_;
+        }
+        __syncthreads();   // workspace rows written by other lanes are read below
+        // backward sweep
+        for (int s = se - 1; s >= sb; --s) {
+            const double* row = wk + (size_t)(s - sb) * SODECL_WROW;
+            const double ts = row[SODECL_NSTAGE * _numeq_];
+            double x[SODECL_CPL], gv[SODECL_CPL];
+#if SODECL_SOLVER == 1
+            // y' = y + f(t, y) dt: lam_s = lam' + J^T (dt lam'), gp += f_p^T (dt lam')
+            double l1[SODECL_CPL];
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                x[c] = comp < _numeq_ ? row[comp] : 0.0;
+                l1[c] = _m_dt_ * lam[c];
+            }
+            sodecl_group_vjp(ts, x, l1, gv, sh_v, sh_l, sh_g[slot], vb, slot, lane, pp, gp);
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) lam[c] = lam[c] + gv[c];
+#else
+            // y' = y + (k1 + 2 k2 + 2 k3 + k4) dt / 6, k_i = f(t_i, x_i),
+            // x2 = y + dt/2 k1, x3 = y + dt/2 k2, x4 = y + dt k3
+            double kb1[SODECL_CPL], kb2[SODECL_CPL], kb3[SODECL_CPL], kb4[SODECL_CPL], yb[SODECL_CPL];
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                kb1[c] = (_m_dt_ / 6.00) * lam[c];
+                kb2[c] = (2.00 * _m_dt_ / 6.00) * lam[c];
+                kb3[c] = kb2[c];
+                kb4[c] = kb1[c];
+                yb[c] = lam[c];
+            }
+#define SODECL_LOAD_STAGE(S)                                                   \
+    _Pragma("unroll") for (int c = 0; c < SODECL_CPL; ++c) {                   \
+        const int comp = lane + c * SODECL_G;                                  \
+        x[c] = comp < _numeq_ ? row[(S) * _numeq_ + comp] : 0.0;               \
+    }
+            SODECL_LOAD_STAGE(3)
+            sodecl_group_vjp(ts + _m_dt_, x, kb4, gv, sh_v, sh_l, sh_g[slot], vb, slot, lane, pp, gp);
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                yb[c] += gv[c];
+                kb3[c] += _m_dt_ * gv[c];
+            }
+            SODECL_LOAD_STAGE(2)
+            sodecl_group_vjp(ts + 0.50 * _m_dt_, x, kb3, gv, sh_v, sh_l, sh_g[slot], vb, slot, lane, pp, gp);
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                yb[c] += gv[c];
+                kb2[c] += (_m_dt_ * 0.50) * gv[c];
+            }
+            SODECL_LOAD_STAGE(1)
+            sodecl_group_vjp(ts + 0.50 * _m_dt_, x, kb2, gv, sh_v, sh_l, sh_g[slot], vb, slot, lane, pp, gp);
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                yb[c] += gv[c];
+                kb1[c] += (_m_dt_ * 0.50) * gv[c];
+            }
+            SODECL_LOAD_STAGE(0)
+            sodecl_group_vjp(ts, x, kb1, gv, sh_v, sh_l, sh_g[slot], vb, slot, lane, pp, gp);
+#pragma unroll
+            for (int c = 0; c < SODECL_CPL; ++c) lam[c] = yb[c] + gv[c];
+#undef SODECL_LOAD_STAGE
+#endif
+            // snapshot of step s: add its cotangent
+            if (s % _numsteps_ == 0) {
+                const long long j = s / _numsteps_;
+#pragma unroll
+                for (int c = 0; c < SODECL_CPL; ++c) {
+                    const int comp = lane + c * SODECL_G;
+                    if (comp < _numeq_) lam[c] += cot[(size_t)j * slab + (size_t)comp * orbits + ob];
+                }
+            }
+        }
+        __syncthreads();   // the next segment overwrites the workspace
+    }
+    if (active) {
+#pragma unroll
+        for (int c = 0; c < SODECL_CPL; ++c) {
+            const int comp = lane + c * SODECL_G;
+            if (comp < _numeq_) grad_y0[(size_t)orbit * _numeq_ + comp] = lam[c];
+        }
+    }
+    __syncthreads();
+    if (active)
+        for (int k = lane; k < _numpar_; k += SODECL_G) grad_p[(size_t)orbit * _numpar_ + k] = gp[k];
+}
+#endif  // adjoint

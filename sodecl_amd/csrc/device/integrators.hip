@@ -262,7 +262,8 @@ SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
 extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
 sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
             double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
-            int orbits, int s0, int nseg, int final_snap)
+            int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
+            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0)
 {
     const int i = blockIdx.x * SODECL_BLOCK + threadIdx.x;
     if (i >= orbits) return;
@@ -279,13 +280,20 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
     double noise[SODECL_NNOI_ALLOC];
 #endif
     long long nu = 0;
+    long long g = gstep0;
     for (int s = 0; s < nseg; ++s) {
         if (traj) {
             double* o = traj + (size_t)(s0 + s) * slab + i;
             SODECL_UNROLL
             for (int m = 0; m < _numeq_; ++m) o[(size_t)m * orbits] = y[m];
         }
-        for (int it = 0; it < _numsteps_; ++it) {
+        for (int it = 0; it < _numsteps_; ++it, ++g) {
+            if (ckpt_y && g % ckpt_every == 0) {
+                const long long k = g / ckpt_every;
+                SODECL_UNROLL
+                for (int m = 0; m < _numeq_; ++m) ckpt_y[(size_t)k * slab + (size_t)m * orbits + i] = y[m];
+                ckpt_t[(size_t)k * orbits + i] = t;
+            }
 #if SODECL_IS_SDE
             // Noise for this step (stochastic_solver_caller.cl:44-66)
             int nc = 0;
@@ -341,11 +349,110 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
 #define SODECL_CPL ((_numeq_ + SODECL_G - 1) / SODECL_G)
 #define SODECL_OPB (SODECL_BLOCK / SODECL_G)
 #define SODECL_XSTRIDE (_numeq_ | 1)
+#define SODECL_NSTAGE (SODECL_SOLVER == 2 ? 4 : 1)
 
+// Writes this lane's components of v into LDS row x.
+SODECL_INLINE void sodecl_publish(double* x, const double (&v)[SODECL_CPL], int lane)
+{
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        if (comp < _numeq_) x[comp] = v[c];
+    }
+}
+
+// Optional copy of a stage vector to a per-orbit workspace row (adjoint).
+SODECL_INLINE void sodecl_store_row(double* row, const double (&v)[SODECL_CPL], int lane)
+{
+    if (!row) return;
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        if (comp < _numeq_) row[comp] = v[c];
+    }
+}
+
+#if SODECL_SOLVER == 1
+// Euler (euler.cl:10-18).  stages: null, or one row receiving y.
+SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (*xs)[SODECL_OPB][SODECL_XSTRIDE],
+                                     int& buf, int slot, int lane, double* pp, double* stages)
+{
+    sodecl_store_row(stages, y, lane);
+    sodecl_publish(xs[buf][slot], y, lane);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        if (comp < _numeq_) {
+            const double f = sode_system_i(comp, t, xs[buf][slot], pp);
+            y[c] = y[c] + f * _m_dt_;
+        }
+    }
+    buf ^= 1;
+}
+#elif SODECL_SOLVER == 2
+// Classic RK4 (rk4.cl:17-64), one LDS stage row per evaluation.  stages:
+// null, or four rows receiving x1 = y, x2, x3, x4.
+SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (*xs)[SODECL_OPB][SODECL_XSTRIDE],
+                                     int& buf, int slot, int lane, double* pp, double* stages)
+{
+    double acc[SODECL_CPL], xn[SODECL_CPL];
+    sodecl_store_row(stages, y, lane);
+    sodecl_publish(xs[buf][slot], y, lane);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        const double f = comp < _numeq_ ? sode_system_i(comp, t, xs[buf][slot], pp) : 0.0;
+        acc[c] = f;
+        xn[c] = y[c] + (_m_dt_ * 0.50) * f;
+    }
+    buf ^= 1;
+    sodecl_store_row(stages ? stages + _numeq_ : nullptr, xn, lane);
+    sodecl_publish(xs[buf][slot], xn, lane);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, xs[buf][slot], pp) : 0.0;
+        acc[c] = acc[c] + 2.00 * f;
+        xn[c] = y[c] + (_m_dt_ * 0.50) * f;
+    }
+    buf ^= 1;
+    sodecl_store_row(stages ? stages + 2 * _numeq_ : nullptr, xn, lane);
+    sodecl_publish(xs[buf][slot], xn, lane);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, xs[buf][slot], pp) : 0.0;
+        acc[c] = acc[c] + 2.00 * f;
+        xn[c] = y[c] + _m_dt_ * f;
+    }
+    buf ^= 1;
+    sodecl_store_row(stages ? stages + 3 * _numeq_ : nullptr, xn, lane);
+    sodecl_publish(xs[buf][slot], xn, lane);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        const double f = comp < _numeq_ ? sode_system_i(comp, t + _m_dt_, xs[buf][slot], pp) : 0.0;
+        y[c] = y[c] + (acc[c] + f) * _m_dt_ / 6.00;
+    }
+    buf ^= 1;
+}
+#endif
+
+// Group-schedule kernel.  Arguments as sodecl_lane, plus optional
+// checkpoints: when ckpt_y is not null, the state (and t) before every global
+// step g with g % ckpt_every == 0 is written to ckpt_y[g/ckpt_every][neq][orbits]
+// and ckpt_t[g/ckpt_every][orbits]; gstep0 is the global index of this
+// launch's first step.
 extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
 sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
              double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
-             int orbits, int s0, int nseg, int final_snap)
+             int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
+             double* __restrict__ ckpt_y, int ckpt_every, long long gstep0)
 {
     __shared__ double sh_x[2][SODECL_OPB][SODECL_XSTRIDE];
     __shared__ double sh_p[SODECL_OPB][SODECL_NPAR_ALLOC];
@@ -375,12 +482,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
     const int calls = (int)ceilf((float)((float)(_numnoi_) * 0.5));
 #endif
     int buf = 0;
-
-#define SODECL_PUBLISH(VALS)                                                  \
-    _Pragma("unroll") for (int c = 0; c < SODECL_CPL; ++c) {                  \
-        const int comp = lane + c * SODECL_G;                                 \
-        if (comp < _numeq_) sh_x[buf][slot][comp] = VALS[c];                  \
-    }
+    long long g = gstep0;
 
     for (int s = 0; s < nseg; ++s) {
         if (traj && active) {
@@ -390,63 +492,17 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
                 if (comp < _numeq_) traj[(size_t)(s0 + s) * slab + (size_t)comp * orbits + orbit] = y[c];
             }
         }
-        for (int it = 0; it < _numsteps_; ++it) {
-#if SODECL_SOLVER == 1
-            // Euler (euler.cl:10-18)
-            SODECL_PUBLISH(y)
-            __syncthreads();
+        for (int it = 0; it < _numsteps_; ++it, ++g) {
+            if (ckpt_y && active && g % ckpt_every == 0) {
+                const long long k = g / ckpt_every;
 #pragma unroll
-            for (int c = 0; c < SODECL_CPL; ++c) {
-                const int comp = lane + c * SODECL_G;
-                if (comp < _numeq_) {
-                    const double f = sode_system_i(comp, t, sh_x[buf][slot], pp);
-                    y[c] = y[c] + f * _m_dt_;
+                for (int c = 0; c < SODECL_CPL; ++c) {
+                    const int comp = lane + c * SODECL_G;
+                    if (comp < _numeq_) ckpt_y[(size_t)k * slab + (size_t)comp * orbits + orbit] = y[c];
                 }
+                if (lane == 0) ckpt_t[(size_t)k * orbits + orbit] = t;
             }
-            buf ^= 1;
-#elif SODECL_SOLVER == 2
-            // Classic RK4 (rk4.cl:17-64), one LDS stage row per evaluation
-            double acc[SODECL_CPL], xn[SODECL_CPL];
-            SODECL_PUBLISH(y)
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < SODECL_CPL; ++c) {
-                const int comp = lane + c * SODECL_G;
-                const double f = comp < _numeq_ ? sode_system_i(comp, t, sh_x[buf][slot], pp) : 0.0;
-                acc[c] = f;
-                xn[c] = y[c] + (_m_dt_ * 0.50) * f;
-            }
-            buf ^= 1;
-            SODECL_PUBLISH(xn)
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < SODECL_CPL; ++c) {
-                const int comp = lane + c * SODECL_G;
-                const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, sh_x[buf][slot], pp) : 0.0;
-                acc[c] = acc[c] + 2.00 * f;
-                xn[c] = y[c] + (_m_dt_ * 0.50) * f;
-            }
-            buf ^= 1;
-            SODECL_PUBLISH(xn)
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < SODECL_CPL; ++c) {
-                const int comp = lane + c * SODECL_G;
-                const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, sh_x[buf][slot], pp) : 0.0;
-                acc[c] = acc[c] + 2.00 * f;
-                xn[c] = y[c] + _m_dt_ * f;
-            }
-            buf ^= 1;
-            SODECL_PUBLISH(xn)
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < SODECL_CPL; ++c) {
-                const int comp = lane + c * SODECL_G;
-                const double f = comp < _numeq_ ? sode_system_i(comp, t + _m_dt_, sh_x[buf][slot], pp) : 0.0;
-                y[c] = y[c] + (acc[c] + f) * _m_dt_ / 6.00;
-            }
-            buf ^= 1;
-#elif SODECL_IS_SDE
+#if SODECL_IS_SDE
             // Noise: Threefry call k of this step goes to lane k of the group;
             // the counter of call k is counter + k, as the reference's
             // sequential +1 per call (stochastic_solver_caller.cl:44-66).
@@ -457,7 +513,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
                 if (2 * k + 1 < _numnoi_) sh_n[buf][slot][2 * k + 1] = z1;
             }
             cnt = cnt + (double)calls;
-            SODECL_PUBLISH(y)
+            sodecl_publish(sh_x[buf][slot], y, lane);
             __syncthreads();
             // Euler-Maruyama (stochastic_euler.cl:10-22)
 #pragma unroll
@@ -465,11 +521,13 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
                 const int comp = lane + c * SODECL_G;
                 if (comp < _numeq_) {
                     const double f = sode_system_i(comp, t, sh_x[buf][slot], pp);
-                    const double g = sode_system_stoch_i(comp, t, sh_x[buf][slot], pp, sh_n[buf][slot]);
-                    y[c] = y[c] + f * _m_dt_ + g;
+                    const double gn = sode_system_stoch_i(comp, t, sh_x[buf][slot], pp, sh_n[buf][slot]);
+                    y[c] = y[c] + f * _m_dt_ + gn;
                 }
             }
             buf ^= 1;
+#else
+            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, nullptr);
 #endif
             t = t + _m_dt_;
         }
@@ -491,6 +549,5 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
 #endif
         }
     }
-#undef SODECL_PUBLISH
 }
 #endif  // group schedule

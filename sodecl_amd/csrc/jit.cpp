@@ -27,7 +27,7 @@
 #include "sodecl_internal.hpp"
 
 namespace {
-#include "_embedded_device.inc"   // kIntegratorsSource, generated from device/*.hip
+#include "_embedded_device.inc"   // kPreludeSource, kIntegratorsSource (device/*.hip)
 }
 
 namespace sodecl_amd {
@@ -135,6 +135,7 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
          "typedef unsigned char uchar;\n";
     o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
     o << "#ifndef M_PI_F\n#define M_PI_F 3.14159265358979323846f\n#endif\n";
+    o << kPreludeSource << "\n";
     // The plugin's functions become host+device functions.
     o << "#pragma clang force_cuda_host_device begin\n";
     o << "#line 1 \"system.cl\"\n";

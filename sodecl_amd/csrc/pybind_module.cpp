@@ -172,6 +172,34 @@ public:
         if (!ok) fail();
     }
 
+    void run_device_checkpointed(uintptr_t t0, uintptr_t y0, uintptr_t params, uintptr_t counters, uintptr_t traj,
+                                 uintptr_t ckpt_t, uintptr_t ckpt_y, int ckpt_every, uintptr_t stream)
+    {
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_run_device_checkpointed(ctx_, (double*)t0, (double*)y0, (const double*)params,
+                                                (double*)counters, (double*)traj, (double*)ckpt_t, (double*)ckpt_y,
+                                                ckpt_every, (void*)stream);
+        }
+        if (!ok) fail();
+    }
+    void adjoint_device(uintptr_t ckpt_t, uintptr_t ckpt_y, uintptr_t params, uintptr_t cot, uintptr_t grad_y0,
+                        uintptr_t grad_p, uintptr_t work, int ckpt_every, uintptr_t stream)
+    {
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_adjoint_device(ctx_, (const double*)ckpt_t, (const double*)ckpt_y, (const double*)params,
+                                       (const double*)cot, (double*)grad_y0, (double*)grad_p, (double*)work,
+                                       ckpt_every, (void*)stream);
+        }
+        if (!ok) fail();
+    }
+    int default_checkpoint_interval() const { return sodecl_default_checkpoint_interval(ctx_); }
+    long checkpoint_count(int every) const { return sodecl_checkpoint_count(ctx_, every); }
+    size_t adjoint_workspace_size(int every) const { return sodecl_adjoint_workspace_size(ctx_, every); }
+
     py::array_t<double> get_counters()
     {
         py::array_t<double> c(orbits_);
@@ -244,6 +272,15 @@ PYBIND11_MODULE(sodecl_interface, m)
         .def("run", &Session::run)
         .def("run_device", &Session::run_device, py::arg("t0"), py::arg("y0"), py::arg("params"),
              py::arg("counters"), py::arg("traj"), py::arg("stream") = 0)
+        .def("run_device_checkpointed", &Session::run_device_checkpointed, py::arg("t0"), py::arg("y0"),
+             py::arg("params"), py::arg("counters"), py::arg("traj"), py::arg("ckpt_t"), py::arg("ckpt_y"),
+             py::arg("ckpt_every"), py::arg("stream") = 0)
+        .def("adjoint_device", &Session::adjoint_device, py::arg("ckpt_t"), py::arg("ckpt_y"), py::arg("params"),
+             py::arg("cot"), py::arg("grad_y0"), py::arg("grad_p"), py::arg("work"), py::arg("ckpt_every"),
+             py::arg("stream") = 0)
+        .def("default_checkpoint_interval", &Session::default_checkpoint_interval)
+        .def("checkpoint_count", &Session::checkpoint_count)
+        .def("adjoint_workspace_size", &Session::adjoint_workspace_size)
         .def("counters", &Session::get_counters)
         .def("newton_updates", &Session::newton_updates)
         .def("launch_config", &Session::launch_config)

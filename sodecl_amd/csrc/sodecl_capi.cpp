@@ -46,7 +46,7 @@ bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what)
     return false;
 }
 
-static ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra)
+ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra)
 {
     ProgramKey k;
     k.solver = ctx->solver;
@@ -219,7 +219,8 @@ LaunchCfg choose_config(const sodecl_ctx* c)
 // Enqueues one launch of the chosen schedule: segments [s0, s0 + nseg) of the
 // trajectory buffer traj (snapshot rows), final snapshot when final_snap.
 int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const double* p, double* cnt, double* traj,
-            long long* newton, int s0, int nseg, int final_snap, hipStream_t stream)
+            long long* newton, int s0, int nseg, int final_snap, hipStream_t stream, double* ckpt_t = nullptr,
+            double* ckpt_y = nullptr, int ckpt_every = 0, long long gstep0 = 0)
 {
     auto prog = get_program(c, cfg.schedule, cfg.group, cfg.block);
     if (!prog) return 0;
@@ -234,7 +235,13 @@ int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const dou
     } else {
         grid = (unsigned)((orbits + cfg.block - 1) / cfg.block);
     }
-    void* args[] = {&t, &y, (void*)&p, &cnt, &traj, &newton, &orbits, &s0, &nseg, &final_snap};
+    if (!ckpt_y || ckpt_every <= 0) {
+        ckpt_y = nullptr;
+        ckpt_t = nullptr;
+        ckpt_every = 1;
+    }
+    void* args[] = {&t, &y, (void*)&p, &cnt, &traj, &newton, &orbits, &s0, &nseg, &final_snap,
+                    &ckpt_t, &ckpt_y, &ckpt_every, &gstep0};
     if (!hip_ok(c, hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)cfg.block, 1, 1, 0, stream, args, nullptr),
                 "hipModuleLaunchKernel"))
         return 0;
@@ -421,7 +428,7 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
             const bool last = s + n >= launches;
             if (k >= 2) hipStreamWaitEvent(c->stream, cdone[b], 0);
             ok = enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, c->d_traj[b], c->d_newton, 0, n,
-                         last ? 1 : 0, c->stream);
+                         last ? 1 : 0, c->stream, nullptr, nullptr, 0, (long long)s * c->ksteps);
             if (!ok) break;
             hipEventRecord(kdone[b], c->stream);
             if (k >= 2) {
@@ -458,6 +465,47 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
 }
 
 }  // namespace
+
+namespace sodecl_amd {
+
+int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,
+                    double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream)
+{
+    if (!c) return 0;
+    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
+    if (!t0 || !y0 || (c->npar > 0 && !params)) return fail(c, "null device pointer");
+    if (c->solver == SODECL_STOCHASTIC_EULER && !counters) return fail(c, "the stochastic solver needs counters");
+    if (ckpt_y && (!ckpt_t || ckpt_every <= 0)) return fail(c, "checkpoints need ckpt_t and ckpt_every > 0");
+    if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
+    const long launches = num_launches(c->tspan, c->dt, c->ksteps);
+    if (launches < 0) return fail(c, "too many kernel launches");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    LaunchCfg cfg = choose_config(c);
+    long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
+    if (launches == 0) {
+        if (traj) return enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, 0, 0, 1, s);
+        return 1;
+    }
+    for (long s0 = 0; s0 < launches; s0 += per) {
+        int n = (int)std::min(per, launches - s0);
+        bool last = s0 + n >= launches;
+        if (!enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, (int)s0, n, last ? 1 : 0, s, ckpt_t, ckpt_y,
+                     ckpt_every, (long long)s0 * c->ksteps))
+            return 0;
+    }
+    return 1;
+}
+
+int launch_config(const sodecl_ctx* c, int* schedule, int* group, int* block)
+{
+    LaunchCfg cfg = choose_config(c);
+    *schedule = cfg.schedule;
+    *group = cfg.group;
+    *block = cfg.block;
+    return 1;
+}
+
+}  // namespace sodecl_amd
 
 extern "C" {
 
@@ -656,27 +704,7 @@ int sodecl_run(sodecl_ctx* c)
 int sodecl_run_device(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,
                       void* stream)
 {
-    if (!c) return 0;
-    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
-    if (!t0 || !y0 || (c->npar > 0 && !params)) return fail(c, "null device pointer");
-    if (c->solver == SODECL_STOCHASTIC_EULER && !counters) return fail(c, "the stochastic solver needs counters");
-    if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
-    const long launches = num_launches(c->tspan, c->dt, c->ksteps);
-    if (launches < 0) return fail(c, "too many kernel launches");
-    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    LaunchCfg cfg = choose_config(c);
-    long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
-    if (launches == 0) {
-        if (traj)
-            return enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, 0, 0, 1, s);
-        return 1;
-    }
-    for (long s0 = 0; s0 < launches; s0 += per) {
-        int n = (int)std::min(per, launches - s0);
-        bool last = s0 + n >= launches;
-        if (!enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, (int)s0, n, last ? 1 : 0, s)) return 0;
-    }
-    return 1;
+    return sodecl_amd::run_device_impl(c, t0, y0, params, counters, traj, nullptr, nullptr, 0, stream);
 }
 
 size_t sodecl_output_size(const sodecl_ctx* c)
@@ -725,27 +753,24 @@ size_t sodecl_compile_only(sodecl_ctx* c, const char* arch)
 {
     if (!c) return 0;
     LaunchCfg cfg = choose_config(c);
-    ProgramKey key;
-    key.solver = c->solver;
-    key.neq = c->neq;
-    key.npar = c->npar;
-    key.nnoi = c->nnoi;
-    key.orbits = c->orbits;
-    key.ksteps = c->ksteps;
-    key.dt = c->dt;
-    key.schedule = cfg.schedule;
-    key.group = cfg.group;
-    key.block = cfg.block;
-    key.newton_max_iter = c->opts.newton_max_iter > 0 ? c->opts.newton_max_iter : 1000;
-    key.fp_contract = c->opts.fp_contract;
-    c->last_program = assemble_program(c->system_source, key, c->features);
-    std::vector<char> code;
-    std::string log;
-    if (!compile_program(c->last_program, arch ? arch : "gfx950", key.fp_contract, code, log)) {
-        fail(c, "Failed to build program executable:\n" + log);
-        return 0;
+    ProgramKey key = sodecl_amd::make_key(c, cfg.schedule, cfg.group, cfg.block, std::string());
+    size_t total = 0;
+    std::vector<std::string> variants = {std::string()};
+    // the adjoint program too, where this configuration supports one
+    if (cfg.schedule == 2 && c->features.vjp_group && (c->solver == SODECL_EULER || c->solver == SODECL_RUNGE_KUTTA))
+        variants.push_back("#define SODECL_ADJOINT 1\n");
+    for (auto& extra : variants) {
+        key.extra_defines = extra;
+        c->last_program = assemble_program(c->system_source, key, c->features);
+        std::vector<char> code;
+        std::string log;
+        if (!compile_program(c->last_program, arch ? arch : "gfx950", key.fp_contract, code, log)) {
+            fail(c, "Failed to build program executable:\n" + log);
+            return 0;
+        }
+        total += code.size();
     }
-    return code.size();
+    return total;
 }
 
 const char* sodecl_program_source(sodecl_ctx* c) { return c ? c->last_program.c_str() : ""; }

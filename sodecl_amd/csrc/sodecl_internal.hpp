@@ -127,8 +127,13 @@ struct sodecl_ctx {
 
 namespace sodecl_amd {
 int fail(sodecl_ctx* ctx, const std::string& msg);
+ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra);
 bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what);
 // Program for the given schedule on the context's device (compiled on first use).
 std::shared_ptr<Program> get_program(sodecl_ctx* ctx, int schedule, int group, int block,
                                      const std::string& extra_defines = std::string());
+int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,
+                    double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream);
+// The schedule, group size and block size runs of this context use.
+int launch_config(const sodecl_ctx* c, int* schedule, int* group, int* block);
 }  // namespace sodecl_amd

@@ -90,3 +90,47 @@ void calc_jacobian_params(double y[_numeq_], double jp[_numeq_][_numpar_], doubl
         jp[i][i + 1] = 1;
     }
 }
+
+#ifdef SODECL_GROUP_API
+/* Vector-Jacobian product of the drift for the adjoint, evaluated by all
+ * lanes of the orbit's group together (x, lam in memory shared by the
+ * group).  With c_i = cos x_i, s_i = sin x_i and the group sums
+ *   A = sum lam_i c_i, B = sum lam_i s_i, C = sum c_i, D = sum s_i,
+ * sin(x_j - x_i) = s_j c_i - c_j s_i gives
+ *   (lam^T df/dx)_m = (K/N) (c_m (A - lam_m C) + s_m (B - lam_m D)),
+ *   lam^T df/dK     = (A D - B C) / N,      lam^T df/dw_m = lam_m,
+ * so each lane does O(N/G) trigonometry and four group reductions instead
+ * of the O(N^2) sums of the drift itself. */
+void sode_system_vjp_group(double t, double x[_numeq_], double lam[_numeq_], double p[_numpar_], double gx[_numeq_],
+                           double gp[_numpar_])
+{
+    const int G = sodecl_group_size();
+    const int l = sodecl_lane();
+    double A = 0, B = 0, C = 0, D = 0;
+    for (int i = l; i < KURA_N; i += G)
+    {
+        double s, c;
+        sincos(x[i], &s, &c);
+        A = A + lam[i] * c;
+        B = B + lam[i] * s;
+        C = C + c;
+        D = D + s;
+    }
+    A = sodecl_group_sum(A);
+    B = sodecl_group_sum(B);
+    C = sodecl_group_sum(C);
+    D = sodecl_group_sum(D);
+    const double kn = p[0] / (double)(KURA_N);
+    for (int m = l; m < KURA_N; m += G)
+    {
+        double s, c;
+        sincos(x[m], &s, &c);
+        gx[m] = kn * (c * (A - lam[m] * C) + s * (B - lam[m] * D));
+        gp[m + 1] = gp[m + 1] + lam[m];
+    }
+    if (l == 0)
+    {
+        gp[0] = gp[0] + (A * D - B * C) / (double)(KURA_N);
+    }
+}
+#endif
