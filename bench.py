@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-orbits", type=int, default=0, help="orbits in the CPU baseline sample")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     return ap.parse_args()
 
 
@@ -84,6 +85,63 @@ def cpu_baseline(cfg, orbits_sample):
                 sample=f"{k} of {cfg['orbits']} orbits, all {steps} steps with trajectory read-back "
                        f"({dt:.2f} s wall)",
                 seconds=dt)
+
+
+def fwd_adjoint_bench(steps):
+    """Kuramoto N=64, RK4, dt=0.01, 2000 steps, 16384 orbits, a snapshot
+    every 10 steps; loss = mean sin/cos misfit against the trajectory of
+    seeded ground-truth (K, w); gradients w.r.t. K, w and theta0 through the
+    discrete adjoint with ceil(sqrt(2000)) = 45-step checkpoints."""
+    import torch
+    from sodecl_amd import configs
+    from sodecl_amd.torch_ops import Integrator
+    cfg = configs.kuramoto_rk4()
+    dev = torch.device("cuda:0")
+    integ = Integrator(cfg["system"], cfg["solver"], cfg["orbits"], cfg["neq"], cfg["npar"], cfg["dt"],
+                       cfg["tspan"], cfg["ksteps"])
+    y0 = torch.tensor(cfg["y0"], device=dev)
+    truth = torch.tensor(cfg["params"], device=dev)
+    rng = torch.Generator(device="cpu").manual_seed(1)
+    p0 = truth.clone()
+    p0[:, 0] *= 1.1
+    p0[:, 1:] += (0.02 * torch.randn(p0[:, 1:].shape, generator=rng, dtype=torch.float64)).to(dev)
+    with torch.no_grad():
+        target = integ(y0, truth)
+    st = torch.sin(target)
+    ct = torch.cos(target)
+
+    def fwd():
+        with torch.no_grad():
+            traj = integ(y0, p0)
+            return ((torch.sin(traj) - st) ** 2 + (torch.cos(traj) - ct) ** 2).mean()
+
+    def fwd_adj():
+        y = y0.clone().requires_grad_(True)
+        p = p0.clone().requires_grad_(True)
+        traj = integ(y, p)
+        loss = ((torch.sin(traj) - st) ** 2 + (torch.cos(traj) - ct) ** 2).mean()
+        loss.backward()
+        return loss, y.grad, p.grad
+
+    fwd()
+    fwd_adj()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fwd()
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, gy, gp = fwd_adj()
+    torch.cuda.synchronize()
+    t_all = (time.perf_counter() - t0) / steps
+    total = cfg["orbits"] * integ.session.launches * cfg["ksteps"]
+    return dict(workload="kuramoto_rk4_adjoint", orbits=cfg["orbits"], neq=cfg["neq"], steps=total // cfg["orbits"],
+                checkpoints=integ.session.checkpoint_count(integ.ckpt_every), ckpt_every=integ.ckpt_every,
+                fwd_value=total / t_fwd, fwd_adjoint_value=total / t_all, unit="orbit-steps/s",
+                fwd_ms=t_fwd * 1e3, fwd_adjoint_ms=t_all * 1e3, cost_ratio=t_all / t_fwd,
+                loss=float(loss), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
 
 
 def main():
@@ -175,6 +233,11 @@ def main():
         plain["bitwise_equal_to_fast_path"] = same if sched[0] == 1 else None
         del pres, got
 
+    adj = None
+    if not args.no_adjoint and args.config == "oculomotor_rk4":
+        log("forward+adjoint (Kuramoto N=64 RK4, 16384 orbits)")
+        adj = fwd_adjoint_bench(max(1, min(args.steps, 3)))
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (reference kernels on host cores)")
@@ -217,6 +280,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "plain_kernel": plain,
+        "fwd_adjoint": adj,
         "speedup_vs_plain": None if plain is None else value / plain["value"],
         "speedup_vs_cpu": None if cpu is None else value / cpu["value"],
         "finite_fraction_final_state": finite,

@@ -148,8 +148,9 @@ long sodecl_num_launches(double tspan, double dt, int ksteps);
 /* Device-resident run (no reference counterpart): all pointers are device
  * memory of the context's device.  t0/y0/params/counters as above (updated
  * in place); traj receives nsnap snapshots [snap][neq][orbits] (may be NULL);
- * stream is a hipStream_t or NULL for the context's stream.  Returns after
- * the work is enqueued; synchronise the stream before reading results. */
+ * stream is a hipStream_t; NULL is HIP's default (null) stream, ordered with
+ * other default-stream work.  Returns after the work is enqueued;
+ * synchronise the stream before reading results. */
 int sodecl_run_device(sodecl_ctx* ctx, double* t0, double* y0, const double* params, double* counters,
                       double* traj, void* stream);
 

@@ -91,7 +91,7 @@ int sodecl_adjoint_device(sodecl_ctx* c, const double* ckpt_t, const double* ckp
     const unsigned grid = (unsigned)((orbits + opb - 1) / opb);
     void* args[] = {(void*)&ckpt_t, (void*)&ckpt_y, (void*)&params, (void*)&cot, &grad_y0, &grad_params,
                     &work, &orbits, &nsteps, &every};
-    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    hipStream_t s = (hipStream_t)stream;   // null: HIP's default stream
     if (!hip_ok(c, hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)block, 1, 1, 0, s, args, nullptr),
                 "hipModuleLaunchKernel(adjoint)"))
         return 0;

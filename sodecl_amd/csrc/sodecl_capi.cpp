@@ -479,7 +479,9 @@ int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params,
     if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
     const long launches = num_launches(c->tspan, c->dt, c->ksteps);
     if (launches < 0) return fail(c, "too many kernel launches");
-    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    // a null stream is HIP's default stream, ordered with the caller's other
+    // default-stream work (PyTorch's current stream, typically)
+    hipStream_t s = (hipStream_t)stream;
     LaunchCfg cfg = choose_config(c);
     long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
     if (launches == 0) {

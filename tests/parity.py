@@ -1,10 +1,10 @@
 """Shared comparison rules for the parity tests.
 
 Tolerance for FP64 trajectories: 1e-10 relative.  An element passes when
-|gpu - oracle| <= 1e-10 * max(|oracle|, scale_eq), scale_eq being the largest
-finite |oracle| of that equation over the compared snapshots and orbits (so
-components crossing zero are judged against their own magnitude).  Non-finite
-values must be non-finite in both.
+|gpu - oracle| <= 1e-10 * max(|oracle|, scale), scale being the largest
+finite |oracle| of that equation of that orbit over the compared snapshots
+(so a component crossing zero is judged against its own magnitude along the
+orbit).  Non-finite values must be non-finite in both.
 """
 import numpy as np
 
@@ -17,7 +17,7 @@ def rel_err(gpu, ref):
     gpu = np.asarray(gpu)
     ref = np.asarray(ref)
     finite = np.isfinite(ref)
-    scale = np.max(np.where(finite, np.abs(ref), 0.0), axis=(0, 2), keepdims=True)
+    scale = np.max(np.where(finite, np.abs(ref), 0.0), axis=0, keepdims=True)
     scale = np.maximum(scale, 1e-300)
     with np.errstate(invalid="ignore"):
         err = np.abs(gpu - ref) / np.maximum(np.abs(ref), scale)

