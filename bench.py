@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-orbits", type=int, default=0, help="orbits in the CPU baseline sample")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched implicit measurement")
     return ap.parse_args()
 
 
@@ -144,6 +145,47 @@ def fwd_adjoint_bench(steps):
                 loss=float(loss), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
 
 
+def batch_bench(solver, reps=1):
+    """Brusselator chains of 16/64/256 cells (1:1:1), 32768 orbits, 500..5000
+    steps each, implicit Euler (3) or midpoint (4) with at most 5 Newton
+    updates per step, all in one persistent launch fed by a work queue."""
+    import numpy as np
+    import torch
+    from sodecl_amd import configs, sodecl_interface as si
+    cfg = configs.brusselator_batch(solver=solver)
+    dev = torch.device("cuda:0")
+    b = si.Batch(cfg["system"], solver, cfg["dt"], cfg["ksteps"], cfg["npar"], cfg["class_neq"], 0,
+                 {"newton_max_iter": cfg["newton_max_iter"]})
+    b.setup(cfg["cls"], cfg["steps"])
+    y0 = torch.tensor(cfg["y0"], device=dev)
+    p = torch.tensor(cfg["params"], device=dev)
+    out = torch.empty((b.output_size(),), dtype=torch.float64, device=dev)
+    nu = torch.zeros(cfg["orbits"], dtype=torch.int64, device=dev)
+    stream = torch.cuda.Stream()
+    b.run_device(y0.data_ptr(), p.data_ptr(), out.data_ptr(), nu.data_ptr(), stream.cuda_stream)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        b.run_device(y0.data_ptr(), p.data_ptr(), out.data_ptr(), nu.data_ptr(), stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    kms = sum(ms) / len(ms)
+    updates = nu.cpu().numpy()
+    neq = np.asarray(cfg["class_neq"])[cfg["cls"]]
+    flops = configs.brusselator_flops(neq, cfg["steps"], updates, solver)
+    total = int(np.sum(cfg["steps"]))
+    return dict(workload=f"brusselator_batch_{'ie' if solver == 3 else 'im'}", orbits=cfg["orbits"],
+                classes_neq=cfg["class_neq"], steps_total=total, kernel_ms=kms, value=total / (kms * 1e-3),
+                unit="orbit-steps/s", newton_updates=int(updates.sum()),
+                mean_updates_per_step=float(updates.sum() / total), achieved_tflops=flops / (kms * 1e-3) / 1e12,
+                frac_fp64_peak=flops / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                finite=bool(torch.isfinite(out).all()))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -238,6 +280,11 @@ def main():
         log("forward+adjoint (Kuramoto N=64 RK4, 16384 orbits)")
         adj = fwd_adjoint_bench(max(1, min(args.steps, 3)))
 
+    batch = None
+    if not args.no_batch and args.config == "oculomotor_rk4":
+        log("batched implicit Brusselator (32768 orbits, mixed sizes and step counts)")
+        batch = [batch_bench(3), batch_bench(4)]
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (reference kernels on host cores)")
@@ -281,6 +328,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_kernel": plain,
         "fwd_adjoint": adj,
+        "batch_implicit": batch,
         "speedup_vs_plain": None if plain is None else value / plain["value"],
         "speedup_vs_cpu": None if cpu is None else value / cpu["value"],
         "finite_fraction_final_state": finite,

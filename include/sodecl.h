@@ -175,6 +175,32 @@ int sodecl_adjoint_device(sodecl_ctx* ctx, const double* ckpt_t, const double* c
                           const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
                           void* stream);
 
+/* ---- Batched integration (no reference counterpart) ---------------------
+ * Orbits of several system sizes ("size classes", each a value of _numeq_)
+ * and individual step counts in one persistent launch, implicit Euler or
+ * implicit midpoint with the reference's Newton iteration.  The plugin must
+ * define sode_system_i and calc_jacobian_band_i; it is compiled once per
+ * size class.  Packed device layouts: orbit o's initial state at
+ * y0[yoff[o]], its steps[o]/ksteps + 1 snapshots [snap][neq_o] at
+ * out[ooff[o]], params[orbits][npar]; newton[orbits] receives the Newton
+ * updates each orbit took. */
+typedef struct sodecl_batch sodecl_batch;
+int sodecl_batch_create(sodecl_batch** batch, const char* system_source, int solver, double dt, int ksteps, int npar,
+                        int nclasses, const int* class_neq);
+void sodecl_batch_destroy(sodecl_batch* batch);
+int sodecl_batch_set_options(sodecl_batch* batch, const sodecl_options* options);
+int sodecl_batch_choose_device(sodecl_batch* batch, uint32_t device);
+/* orbit_class[orbits] indexes class_neq; orbit_steps[orbits] are multiples of ksteps. */
+int sodecl_batch_setup(sodecl_batch* batch, int orbits, const int* orbit_class, const int* orbit_steps);
+size_t sodecl_batch_state_size(const sodecl_batch* batch);
+size_t sodecl_batch_output_size(const sodecl_batch* batch);
+int sodecl_batch_offsets(const sodecl_batch* batch, long long* yoff, long long* ooff);
+int sodecl_batch_run_device(sodecl_batch* batch, const double* y0, const double* params, double* out,
+                            long long* newton, void* stream);
+size_t sodecl_batch_compile_only(sodecl_batch* batch, const char* arch);
+const char* sodecl_batch_program_source(const sodecl_batch* batch);
+const char* sodecl_batch_last_error(const sodecl_batch* batch);
+
 /* Wall time of the last sodecl_run segment loop in seconds, and the kernel
  * time of the last run measured with HIP events on the launch stream. */
 double sodecl_last_runtime(const sodecl_ctx* ctx);

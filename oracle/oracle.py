@@ -40,6 +40,8 @@ class _Config(ctypes.Structure):
         ("tspan", ctypes.c_double),
         ("newton_maxit", ctypes.c_int),
         ("nthreads", ctypes.c_int),
+        ("nlaunch", ctypes.c_long),
+        ("use_band", ctypes.c_int),
     ]
 
 
@@ -79,19 +81,20 @@ class Oracle:
         L.orc_counter_word.argtypes = [ctypes.c_double]
 
     def run(self, solver, y0, params, dt, tspan, ksteps, t0=None, counters=None, nnoi=0,
-            newton_maxit=1000, orbit_range=None, nthreads=0, want_output=True):
-        """Integrates every orbit; returns dict(out, y, t, counters, newton_updates, launches)."""
+            newton_maxit=1000, orbit_range=None, nthreads=0, want_output=True, nlaunch=-1, use_band=False):
+        """Integrates every orbit; returns dict(out, y, t, counters, newton_updates, launches).
+        nlaunch >= 0 fixes the number of ksteps segments instead of tspan."""
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
         params = np.ascontiguousarray(params, dtype=np.float64)
         orbits, neq = y0.shape
         npar = params.shape[1]
         c = _Config(SOLVERS[solver] if isinstance(solver, str) else int(solver), neq, npar, nnoi,
-                    orbits, ksteps, dt, tspan, newton_maxit, nthreads)
+                    orbits, ksteps, dt, tspan, newton_maxit, nthreads, nlaunch, int(use_band))
         ob, oc = (0, orbits) if orbit_range is None else orbit_range
         t = np.zeros(orbits) if t0 is None else np.array(t0, dtype=np.float64)
         y = y0.copy()
         cnt = None if counters is None else np.array(counters, dtype=np.float64)
-        launches = self.lib.orc_num_launches(tspan, dt, ksteps)
+        launches = nlaunch if nlaunch >= 0 else self.lib.orc_num_launches(tspan, dt, ksteps)
         out = np.empty((launches + 1, neq, oc)) if want_output else None
         nu = np.zeros(orbits, dtype=np.int64)
         r = self.lib.orc_run(ctypes.byref(c), ob, oc, _ptr(t), _ptr(y), _ptr(params), _ptr(cnt),
@@ -109,7 +112,7 @@ class Oracle:
         cot = np.ascontiguousarray(cot, dtype=np.float64)
         orbits, neq = y0.shape
         npar = params.shape[1]
-        c = _Config(SOLVERS[solver], neq, npar, 0, orbits, ksteps, dt, tspan, 1000, nthreads)
+        c = _Config(SOLVERS[solver], neq, npar, 0, orbits, ksteps, dt, tspan, 1000, nthreads, -1, 0)
         ob, oc = (0, orbits) if orbit_range is None else orbit_range
         gy = np.zeros((oc, neq))
         gp = np.zeros((oc, npar))

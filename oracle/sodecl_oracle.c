@@ -68,6 +68,12 @@ int orc_newton_maxit = 1000;   /* ie.cl:69, im.cl:65 */
 void sode_system_stoch(double t, double* y, double* stoch, double* p, double* noise) __attribute__((weak));
 void calc_jacobian(double* y, double (*jac)[orc_numeq], double* p) __attribute__((weak));
 void calc_jacobian_params(double* y, double (*jp)[orc_numpar], double* p) __attribute__((weak));
+void calc_jacobian_band(double* y, double* lower, double* diag, double* upper, double* p) __attribute__((weak));
+
+/* 1: build the Newton matrix from calc_jacobian_band when the plugin has it
+ * (the same three diagonals the tridiagonal solve reads from the dense
+ * Jacobian; tests check both paths give identical bits). */
+int orc_use_band = 0;
 
 enum { ORC_SEULER = 0, ORC_EULER = 1, ORC_RK4 = 2, ORC_IE = 3, ORC_IM = 4 };
 
@@ -198,6 +204,16 @@ static void orc_newton_matrix(double* g, double* p, double sign, double dt,
                               double* lower, double* diag, double* upper)
 {
     int n = _numeq_;
+    if (orc_use_band && calc_jacobian_band) {
+        calc_jacobian_band(g, lower, diag, upper, p);
+        for (int i = 0; i < n; i++) {
+            lower[i] = (i + 1 < n) ? lower[i] * sign * dt : 0;
+            upper[i] = (i + 1 < n) ? upper[i] * sign * dt : 0;
+            diag[i] = diag[i] * sign * dt;
+            diag[i] = diag[i] + 1;
+        }
+        return;
+    }
     double (*jac)[n] = malloc(sizeof(double) * (size_t)n * (size_t)n);
     calc_jacobian(g, jac, p);
     for (int i = 0; i < n; i++) {
@@ -305,6 +321,8 @@ typedef struct {
     double dt, tspan;
     int newton_maxit;
     int nthreads;
+    long nlaunch;      /* >= 0: number of ksteps segments, overriding tspan */
+    int use_band;      /* build Newton matrices from calc_jacobian_band */
 } orc_config;
 
 /* Integrates one orbit for one segment of ksteps steps, exactly as one
@@ -368,6 +386,7 @@ static void orc_set_globals(const orc_config* c)
     orc_numsteps = c->ksteps;
     orc_kdt = orc_kernel_dt(c->dt);
     orc_newton_maxit = c->newton_maxit > 0 ? c->newton_maxit : 1000;
+    orc_use_band = c->use_band;
 }
 
 /*
@@ -392,7 +411,7 @@ long orc_run(const orc_config* c, int orbit_begin, int orbit_count, double* t0, 
     if (c->solver == ORC_SEULER && (c->nnoi <= 0 || counters == NULL || !sode_system_stoch)) return -1;
     if ((c->solver == ORC_IE || c->solver == ORC_IM) && !calc_jacobian) return -1;
     orc_set_globals(c);
-    long launches = orc_num_launches(c->tspan, c->dt, c->ksteps);
+    long launches = c->nlaunch >= 0 ? c->nlaunch : orc_num_launches(c->tspan, c->dt, c->ksteps);
     if (launches < 0) return -1;
     int n = c->neq;
     int ob = orbit_begin, oc = orbit_count;
@@ -490,7 +509,7 @@ long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_co
     if (!calc_jacobian || (c->npar > 0 && !calc_jacobian_params)) return -1;
     if (c->solver != ORC_EULER && c->solver != ORC_RK4) return -1;
     orc_set_globals(c);
-    long launches = orc_num_launches(c->tspan, c->dt, c->ksteps);
+    long launches = c->nlaunch >= 0 ? c->nlaunch : orc_num_launches(c->tspan, c->dt, c->ksteps);
     int n = c->neq, np = c->npar, ncol = n + np;
     double dt = _m_dt_;
 #ifdef _OPENMP

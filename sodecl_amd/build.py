@@ -21,7 +21,7 @@ ARCH = os.environ.get("SODECL_AMD_ARCH", "gfx950")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 # Every HIP compile: no packed FP32 instructions.
 NO_PK_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-DEVICE_SOURCES = ["integrators.hip", "adjoint.hip", "batch.hip"]
+DEVICE_SOURCES = ["integrators.hip", "adjoint.hip"]
 HOST_SOURCES = ["jit.cpp", "sodecl_capi.cpp", "sodeclmgr.cpp", "adjoint.cpp", "batch.cpp"]
 
 
@@ -49,7 +49,9 @@ def embed_device_sources():
     """device/*.hip -> _embedded_device.inc (C++ raw strings): the prelude
     emitted before the system source and the integrators emitted after it."""
     out = os.path.join(CSRC, "_embedded_device.inc")
-    new = _raw_string("kPreludeSource", ["prelude.hip"]) + _raw_string("kIntegratorsSource", DEVICE_SOURCES)
+    new = (_raw_string("kPreludeSource", ["prelude.hip"]) + _raw_string("kIntegratorsSource", DEVICE_SOURCES) +
+           _raw_string("kBatchClassSource", ["batch_class.hip"]) +
+           _raw_string("kBatchKernelSource", ["batch_kernel.hip"]))
     if not os.path.exists(out) or open(out).read() != new:
         with open(out, "w") as f:
             f.write(new)

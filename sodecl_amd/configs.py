@@ -109,3 +109,34 @@ def warm_jit_cache(arch="gfx950"):
         s = sodecl_interface.Session(os.path.join(SYSTEMS, name), solver, orbits, neq, npar, nnoi, dt, tspan,
                                      ksteps, 0, opts)
         s.compile_only(arch)
+
+
+def brusselator_batch(orbits=32768, seed=0, solver=3, cells=(16, 64, 256), dt=0.01, ksteps=50, smin=500,
+                      smax=5000, newton_max_iter=5):
+    """Brusselator chains of 16/64/256 cells (neq 32/128/512) in equal
+    shares, each orbit with its own step count drawn uniformly from the
+    multiples of ksteps in [smin, smax]; A = 1, B ~ U(2, 4), a ~ U(0.01, 1).
+    Packed state: orbit o's initial state at y0[yoff[o]]."""
+    rng = np.random.default_rng(seed)
+    cls = rng.permutation(np.arange(orbits) % len(cells)).astype(np.int32)
+    steps = (ksteps * rng.integers(smin // ksteps, smax // ksteps + 1, orbits)).astype(np.int32)
+    B = rng.uniform(2.0, 4.0, orbits)
+    a = rng.uniform(0.01, 1.0, orbits)
+    params = np.stack([np.ones(orbits), B, a], 1)
+    class_neq = [2 * c for c in cells]
+    y0 = np.concatenate([brusselator_initial(cells[c], B[o]) for o, c in enumerate(cls)])
+    return dict(name="brusselator_batch", system=os.path.join(SYSTEMS, "brusselator.cl"), solver=solver,
+                class_neq=class_neq, cls=cls, steps=steps, params=params, y0=y0, dt=dt, ksteps=ksteps, npar=3,
+                orbits=orbits, newton_max_iter=newton_max_iter)
+
+
+def brusselator_flops(neq, steps, updates, solver):
+    """Algorithmic FP64 flops of the batched Brusselator run: per state and
+    Newton update 32.5 (implicit Euler) / 34.5 (implicit midpoint: + the
+    midpoint), per state and step 14.5 / 16.5 for the final residual check
+    (right-hand side 9.5, residual 3, norm 2; band Jacobian with scaling 9,
+    tridiagonal solve 8, update 1)."""
+    neq = np.asarray(neq, dtype=np.float64)
+    if solver == 3:
+        return float(np.sum(neq * (32.5 * np.asarray(updates) + 14.5 * np.asarray(steps))))
+    return float(np.sum(neq * (34.5 * np.asarray(updates) + 16.5 * np.asarray(steps))))

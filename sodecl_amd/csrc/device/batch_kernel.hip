@@ -1,0 +1,44 @@
+// Persistent batched integrator: orbits of different system sizes and step
+// counts in one launch.  Each wavefront takes the next orbit from a device
+// work queue (an atomic index into a longest-first order the host builds
+// from neq * steps) and integrates it to the end with the routine of its
+// size class (batch_class.hip, one instance per class); every wave exits
+// once the queue is drained.
+//
+// Packed layouts: the initial state of orbit o at y0[yoff[o] .. + neq_o];
+// its snapshots [snap][neq_o] at out[ooff[o] ...], steps[o] / ksteps + 1 of
+// them; params[orbits][npar]; newton[o] receives the Newton updates taken.
+
+extern "C" __global__ void __launch_bounds__(SODECL_BG)
+sodecl_batch(const int* __restrict__ order, const int* __restrict__ cls, const int* __restrict__ steps,
+             const long long* __restrict__ yoff, const long long* __restrict__ ooff, const double* __restrict__ y0,
+             const double* __restrict__ params, double* __restrict__ out, long long* __restrict__ newton,
+             int* __restrict__ head, int nitems)
+{
+    extern __shared__ double sodecl_lds[];
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0) item = atomicAdd(head, 1);
+        item = __shfl(item, 0, SODECL_BG);
+        if (item >= nitems) break;
+        const int o = order[item];
+        const double* yo = y0 + yoff[o];
+        const double* po = params + (size_t)o * _numpar_;
+        double* oo = out + ooff[o];
+        long long nu = 0;
+        switch (cls[o]) {
+            SODECL_BATCH_CASES
+        default:
+            break;
+        }
+        if (threadIdx.x == 0) newton[o] = nu;
+        __syncthreads();   // the next orbit reuses the LDS rows
+    }
+}
+
+// Resets the work-queue head (a kernel rather than a memset so a captured
+// launch sequence replays it correctly).
+extern "C" __global__ void sodecl_batch_reset(int* head)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *head = 0;
+}

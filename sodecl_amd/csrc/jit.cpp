@@ -146,6 +146,52 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     return o.str();
 }
 
+std::string assemble_batch_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f,
+                                   const std::vector<int>& class_neq)
+{
+    std::ostringstream o;
+    o << "// SODECL batched program assembled for hipRTC\n";
+    o << "#define _numnoi_ 0\n";
+    o << "#define _listsize_ " << k.orbits << "\n";
+    o << "#define _numsteps_ " << k.ksteps << "\n";
+    o << "#define _m_dt_ " << dt_literal(k.dt) << "\n";
+    o << "#define _numpar_ " << k.npar << "\n";
+    o << "#define _epsilon1_ 1e-07\n";
+    o << "#define SODECL_HIP 1\n";
+    o << "#define SODECL_SOLVER " << k.solver << "\n";
+    o << "#define SODECL_SCHEDULE 2\n";
+    o << "#define SODECL_GROUP 64\n";
+    o << "#define SODECL_BG 64\n";
+    o << "#define SODECL_BLOCK 64\n";
+    o << "#define SODECL_NEWTON_MAXIT " << k.newton_max_iter << "\n";
+    o << "#define SODECL_INLINE __device__ __attribute__((always_inline)) inline\n";
+    for (const char* kw : {"__kernel", "__global", "__private"})
+        o << "#ifndef " << kw << "\n#define " << kw << "\n#endif\n";
+    o << "#ifndef __local\n#define __local\n#endif\n#ifndef __constant\n#define __constant const\n#endif\n";
+    o << "typedef unsigned long ulong;\ntypedef unsigned int uint;\ntypedef unsigned short ushort;\n"
+         "typedef unsigned char uchar;\n";
+    o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
+    o << kPreludeSource << "\n";
+    (void)f;
+    // one namespace per size class: the plugin and the per-class solver
+    for (size_t c = 0; c < class_neq.size(); ++c) {
+        o << "namespace sodecl_cls" << c << " {\n";
+        o << "#define _numeq_ " << class_neq[c] << "\n";
+        o << "#pragma clang force_cuda_host_device begin\n";
+        o << "#line 1 \"system.cl\"\n" << system_source << "\n";
+        o << "#pragma clang force_cuda_host_device end\n";
+        o << kBatchClassSource << "\n";
+        o << "#undef _numeq_\n";
+        o << "}  // namespace sodecl_cls" << c << "\n";
+    }
+    o << "#define SODECL_BATCH_CASES";
+    for (size_t c = 0; c < class_neq.size(); ++c)
+        o << " case " << c << ": nu = sodecl_cls" << c << "::sodecl_batch_orbit(yo, po, oo, steps[o], sodecl_lds); break;";
+    o << "\n";
+    o << kBatchKernelSource << "\n";
+    return o.str();
+}
+
 hipFunction_t Program::get(const std::string& name)
 {
     auto it = functions.find(name);

@@ -7,7 +7,9 @@
 #include <pybind11/stl.h>
 
 #include <cstring>
+#include <fstream>
 #include <iostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 
@@ -251,6 +253,76 @@ private:
     darr y0_, params_, t0_;
 };
 
+// Batched implicit integration of orbits of several sizes (sodecl_batch_*).
+class Batch {
+public:
+    Batch(const std::string& system, int solver, double dt, int ksteps, int npar, std::vector<int> class_neq,
+          int device, py::dict options, bool system_is_source)
+    {
+        std::string src = system;
+        if (!system_is_source) {
+            std::ifstream f(system);
+            if (!f) throw std::runtime_error("Error opening file " + system);
+            std::ostringstream ss;
+            ss << f.rdbuf();
+            src = ss.str();
+        }
+        if (!sodecl_batch_create(&b_, src.c_str(), solver, dt, ksteps, npar, (int)class_neq.size(),
+                                 class_neq.data()))
+            fail();
+        sodecl_options o;
+        sodecl_default_options(&o);
+        apply_options(o, options);
+        sodecl_batch_set_options(b_, &o);
+        device_ = device;
+    }
+    ~Batch() { sodecl_batch_destroy(b_); }
+    void setup(py::array_t<int, py::array::c_style | py::array::forcecast> cls,
+               py::array_t<int, py::array::c_style | py::array::forcecast> steps)
+    {
+        if (cls.size() != steps.size()) throw std::invalid_argument("classes and steps differ in length");
+        if (!sodecl_batch_choose_device(b_, (uint32_t)device_)) fail();
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_batch_setup(b_, (int)cls.size(), cls.data(), steps.data());
+        }
+        if (!ok) fail();
+        orbits_ = (int)cls.size();
+    }
+    py::tuple offsets()
+    {
+        py::array_t<long long> y(orbits_), o(orbits_);
+        if (!sodecl_batch_offsets(b_, y.mutable_data(), o.mutable_data())) fail();
+        return py::make_tuple(y, o);
+    }
+    size_t state_size() const { return sodecl_batch_state_size(b_); }
+    size_t output_size() const { return sodecl_batch_output_size(b_); }
+    void run_device(uintptr_t y0, uintptr_t params, uintptr_t out, uintptr_t newton, uintptr_t stream)
+    {
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_batch_run_device(b_, (const double*)y0, (const double*)params, (double*)out,
+                                         (long long*)newton, (void*)stream);
+        }
+        if (!ok) fail();
+    }
+    size_t compile_only(const std::string& arch)
+    {
+        size_t n = sodecl_batch_compile_only(b_, arch.c_str());
+        if (!n) fail();
+        return n;
+    }
+    std::string program_source() { return sodecl_batch_program_source(b_); }
+
+private:
+    [[noreturn]] void fail() { throw std::runtime_error(sodecl_batch_last_error(b_)); }
+    sodecl_batch* b_ = nullptr;
+    int device_ = 0;
+    int orbits_ = 0;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(sodecl_interface, m)
@@ -292,4 +364,18 @@ PYBIND11_MODULE(sodecl_interface, m)
         .def("handle", &Session::handle)
         .def_property_readonly("nsnap", &Session::nsnap)
         .def_property_readonly("launches", &Session::launches);
+    py::class_<Batch>(m, "Batch")
+        .def(py::init<const std::string&, int, double, int, int, std::vector<int>, int, py::dict, bool>(),
+             py::arg("system"), py::arg("solver"), py::arg("dt"), py::arg("ksteps"), py::arg("npar"),
+             py::arg("class_neq"), py::arg("device") = 0, py::arg("options") = py::dict(),
+             py::arg("system_is_source") = false)
+        .def("setup", &Batch::setup, py::arg("classes"), py::arg("steps"))
+        .def("offsets", &Batch::offsets)
+        .def("state_size", &Batch::state_size)
+        .def("output_size", &Batch::output_size)
+        .def("run_device", &Batch::run_device, py::arg("y0"), py::arg("params"), py::arg("out"), py::arg("newton"),
+             py::arg("stream") = 0)
+        .def("compile_only", &Batch::compile_only, py::arg("arch") = "gfx950")
+        .def("program_source", &Batch::program_source)
+        ;
 }

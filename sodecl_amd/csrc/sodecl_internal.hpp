@@ -47,6 +47,10 @@ std::string dt_literal(double dt);
 // Program text handed to hipRTC: macro prelude, OpenCL-C compatibility,
 // the user system source, the integrator templates.
 std::string assemble_program(const std::string& system_source, const ProgramKey& key, const PluginFeatures& f);
+// Program of the batched implicit integrator: the plugin and the per-class
+// solver once per system size in class_neq, then the persistent kernel.
+std::string assemble_batch_program(const std::string& system_source, const ProgramKey& key, const PluginFeatures& f,
+                                   const std::vector<int>& class_neq);
 
 // A compiled and loaded program.
 struct Program {
