@@ -48,16 +48,38 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=0, help="orbits in the CPU baseline sample")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched implicit measurement")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run the ahead-of-time streaming copy kernel once over this many MiB "
+                         "(reference point for HBM byte counters)")
     return ap.parse_args()
 
 
 def traffic_from_profiles(name):
-    """HBM bytes per launch measured with rocprofv3 counters, if committed."""
-    path = os.path.join(HERE, "profiles", f"{name}_traffic.json")
+    """HBM bytes per launch of the integration kernel, from the committed
+    rocprofv3 counter summary (a line 'traffic_bytes_per_launch: N')."""
+    path = os.path.join(HERE, "profiles", f"{name}_hbm_traffic.md")
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].strip())
     return None
+
+
+def calibration_copy(mib):
+    """One launch of the library's streaming copy kernel over mib MiB."""
+    import ctypes
+    import torch
+    import sodecl_amd
+    lib = ctypes.CDLL(sodecl_amd.library_path())
+    lib.sodecl_copy_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    n = mib * (1 << 20)
+    src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    assert lib.sodecl_copy_device(dst.data_ptr(), src.data_ptr(), n, None) == 1
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
 
 
 def cpu_baseline(cfg, orbits_sample):
@@ -274,6 +296,10 @@ def main():
         same = bool(np.array_equal(np.nan_to_num(pres, nan=1e300), np.nan_to_num(got, nan=1e300)))
         plain["bitwise_equal_to_fast_path"] = same if sched[0] == 1 else None
         del pres, got
+
+    if args.calibrate_copy_mb:
+        log(f"calibration copy of {args.calibrate_copy_mb} MiB")
+        calibration_copy(args.calibrate_copy_mb)
 
     adj = None
     if not args.no_adjoint and args.config == "oculomotor_rk4":

@@ -69,11 +69,13 @@ def oracle_run(cfg, system):
 
 def small_oculomotor(solver, orbits=256, seed=5):
     """Randomised oculomotor orbits at step sizes where every orbit is well
-    conditioned: the oracle built with and without fused multiply-add agrees
-    to 1e-15 on all of them (explicit Euler needs dt=2e-6 for that; at 1e-5
-    some parameter draws sit at its stability edge and two valid roundings
-    of the reference arithmetic already disagree at O(1))."""
-    if solver in (0, 1):
+    conditioned: the oracle built with and without fused multiply-add, and
+    from initial states moved by one ulp, agrees with itself to 1e-13 on all
+    of them.  The explicit solvers need dt=2e-6 for that: at 1e-5 some
+    parameter draws sit near their stability edge and two valid roundings of
+    the reference arithmetic already disagree by up to O(1) (Euler) or 1e-10
+    (RK4)."""
+    if solver in (0, 1, 2):
         cfg = configs.oculomotor_rk4(orbits=orbits, seed=seed, dt=2e-6, tspan=0.01, ksteps=100)
     else:
         cfg = configs.oculomotor_rk4(orbits=orbits, seed=seed, dt=1e-5, tspan=0.02, ksteps=50)
