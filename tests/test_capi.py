@@ -1,0 +1,146 @@
+"""CPU tests of the native library: exported symbols, host logic, program
+assembly and offline gfx950 compilation of every solver and schedule.
+No GPU is touched (no kernel is launched)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sodecl_amd
+from sodecl_amd import sodecl_interface as si
+from oracle.oracle import num_launches as oracle_num_launches
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sodecl.h")
+REF = "/root/reference"
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b(sodecl_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_function():
+    lib = ctypes.CDLL(sodecl_amd.library_path())
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_num_launches_matches_oracle_rule():
+    for tspan, dt, ks in [(0.1, 1e-6, 400), (0.1, 1e-8, 40000), (20.0, 1e-3, 100), (10.0, 0.01, 10),
+                          (1.0, 0.3, 1), (0.0, 0.1, 1), (50, 0.05, 40), (0.3, 0.1, 1)]:
+        assert si.num_launches(tspan, dt, ks) == oracle_num_launches(tspan, dt, ks)
+
+
+def test_program_carries_reference_macro_contract():
+    s = si.Session(sodecl_amd.system_path("oculomotor"), 3, 17, 6, 6, 0, 1.23456789e-4, 1.0, 7, 0, {})
+    s.compile_only("gfx950")
+    src = s.program_source()
+    for line in ["#define _numnoi_ 0", "#define _listsize_ 17", "#define _numsteps_ 7", "#define _m_dt_ 0.000123457",
+                 "#define _numeq_ 6", "#define _numpar_ 6", "#define _epsilon1_ 1e-07"]:
+        assert line in src, line
+    assert src.index("#define _numnoi_") < src.index("void sode_system(") < src.index("\nsodecl_lane(")
+
+
+def test_feature_detection_ignores_comments():
+    src = "/* sode_system_i( */\n// calc_jacobian_band(\nvoid sode_system(double t, double y[], double yo[], double p[]) {}\n"
+    s = si.Session(src, 1, 4, 1, 1, 0, 0.1, 1.0, 1, 0, {"schedule": "group"}, True)
+    s.compile_only("gfx950")
+    assert "#define SODECL_HAS_COMPONENT 0" in s.program_source()
+    assert "#define SODECL_HAS_JACOBIAN_BAND 0" in s.program_source()
+
+
+CASES = [
+    ("oculomotor", 1, 6, 6, 0, "lane"), ("oculomotor", 2, 6, 6, 0, "lane"), ("oculomotor", 3, 6, 6, 0, "lane"),
+    ("oculomotor", 4, 6, 6, 0, "lane"), ("oculomotor", 0, 6, 8, 2, "lane"),
+    ("kuramoto", 1, 10, 11, 0, "group"), ("kuramoto", 2, 64, 65, 0, "group"), ("kuramoto", 0, 32, 65, 32, "group"),
+    ("kuramoto", 3, 8, 9, 0, "lane"), ("kuramoto", 4, 8, 9, 0, "lane"), ("kuramoto", 2, 10, 11, 0, "lane"),
+    ("brusselator", 3, 32, 3, 0, "lane"), ("brusselator", 4, 16, 3, 0, "lane"),
+]
+
+
+@pytest.mark.parametrize("system,solver,neq,npar,nnoi,schedule", CASES)
+def test_offline_compile_gfx950(system, solver, neq, npar, nnoi, schedule):
+    s = si.Session(sodecl_amd.system_path(system), solver, 1000, neq, npar, nnoi, 1e-3, 1.0, 10, 0,
+                   {"schedule": schedule})
+    assert s.compile_only("gfx950") > 1000
+
+
+@pytest.mark.parametrize("solver", [3, 4])
+def test_offline_compile_batch(solver):
+    b = si.Batch(sodecl_amd.system_path("brusselator"), solver, 0.01, 50, 3, [32, 128, 512], 0,
+                 {"newton_max_iter": 5})
+    assert b.compile_only("gfx950") > 1000
+    src = b.program_source()
+    assert len(re.findall(r"namespace sodecl_cls\d+ \{", src)) == 3
+
+
+def test_build_error_is_reported():
+    bad = "void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_]) { yout[0] = ; }"
+    s = si.Session(bad, 1, 4, 1, 1, 0, 0.1, 1.0, 1, 0, {}, True)
+    with pytest.raises(RuntimeError, match="Failed to build"):
+        s.compile_only("gfx950")
+
+
+def test_invalid_configuration_rejected():
+    with pytest.raises(RuntimeError, match="solver"):
+        si.Session(sodecl_amd.system_path("oculomotor"), 9, 4, 6, 6, 0, 0.1, 1.0, 1, 0, {})
+    with pytest.raises(RuntimeError):
+        si.Session(sodecl_amd.system_path("oculomotor"), 1, 0, 6, 6, 0, 0.1, 1.0, 1, 0, {})
+    with pytest.raises(RuntimeError, match="opening"):
+        si.Session("/nonexistent/system.cl", 1, 4, 6, 6, 0, 0.1, 1.0, 1, 0, {})
+    with pytest.raises(ValueError, match="unknown option"):
+        si.Session(sodecl_amd.system_path("oculomotor"), 1, 4, 6, 6, 0, 0.1, 1.0, 1, 0, {"bogus": 1})
+
+
+def test_invalid_solver_in_sodeclcall_returns_empty():
+    out = si.sodeclcall(np.zeros(2), np.zeros(12), np.ones(12), 0, 0, sodecl_amd.system_path("oculomotor"), 7, 2, 6,
+                        6, 0, 1e-3, 0.01, 10, 0)
+    assert out.size == 0
+
+
+def test_data_file_readers(tmp_path):
+    """read_binary_data_from_file / read_data_from_file (sodeclmgr.cpp:839-914)."""
+    lib = ctypes.CDLL(sodecl_amd.library_path())
+    for name in ("sodecl_read_binary_data_from_file", "sodecl_read_data_from_file"):
+        getattr(lib, name).argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)]
+    vals = np.array([1.5, -2.25, 3e-7, 4e10])
+    (tmp_path / "d.bin").write_bytes(vals.tobytes())
+    (tmp_path / "d.txt").write_text("1.5 -2.25\n\n3e-7   4e10\n")
+    for fname, fn in (("d.bin", lib.sodecl_read_binary_data_from_file), ("d.txt", lib.sodecl_read_data_from_file)):
+        out = np.zeros(8)
+        n = ctypes.c_size_t()
+        assert fn(str(tmp_path / fname).encode(), out.ctypes.data, 8, ctypes.byref(n)) == 1
+        assert n.value == 4
+        np.testing.assert_array_equal(out[:4], vals)
+    out = np.zeros(2)
+    n = ctypes.c_size_t()
+    assert lib.sodecl_read_data_from_file(str(tmp_path / "missing.txt").encode(), out.ctypes.data, 2,
+                                          ctypes.byref(n)) == 0
+
+
+def test_adjoint_workspace_and_checkpoints():
+    s = si.Session(sodecl_amd.system_path("kuramoto"), 2, 16384, 64, 65, 0, 0.01, 20.0, 10, 0,
+                   {"schedule": "group"})
+    assert s.default_checkpoint_interval() == 45          # ceil(sqrt(2000))
+    assert s.checkpoint_count(45) == 45                   # ceil(2000 / 45)
+    assert s.adjoint_workspace_size(45) == 16384 * 45 * (4 * 64 + 1)
+    lane = si.Session(sodecl_amd.system_path("oculomotor"), 2, 16, 6, 6, 0, 0.01, 1.0, 10, 0, {})
+    assert lane.adjoint_workspace_size(10) == 0
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "examples")), reason="reference tree not present")
+@pytest.mark.parametrize("name,solver,neq,npar,nnoi", [("oculomotor.cl", 2, 6, 6, 0), ("oculomotor.cl", 3, 6, 6, 0),
+                                                       ("oculomotor.cl", 0, 6, 8, 2), ("kuramoto.cl", 1, 5, 6, 0),
+                                                       ("kuramoto.cl", 0, 5, 11, 5)])
+def test_reference_example_plugins_compile_unchanged(name, solver, neq, npar, nnoi):
+    """The reference's own system sources go through the JIT as they are."""
+    s = si.Session(os.path.join(REF, "examples", name), solver, 64, neq, npar, nnoi, 1e-3, 1.0, 10, 0, {})
+    assert s.compile_only("gfx950") > 1000
