@@ -144,3 +144,27 @@ def test_reference_example_plugins_compile_unchanged(name, solver, neq, npar, nn
     """The reference's own system sources go through the JIT as they are."""
     s = si.Session(os.path.join(REF, "examples", name), solver, 64, neq, npar, nnoi, 1e-3, 1.0, 10, 0, {})
     assert s.compile_only("gfx950") > 1000
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "interfaces", "python", "sodecl.cpp")),
+                    reason="reference tree not present")
+def test_reference_pybind_binding_builds_against_sodeclmgr(tmp_path):
+    """The reference's own binding (interfaces/python/sodecl.cpp), with only
+    its include switched to sodeclmgr.hpp, compiles and links against this
+    library: the sodeclmgr surface it uses is all there."""
+    import subprocess
+    import sysconfig
+
+    import pybind11
+    src = open(os.path.join(REF, "interfaces", "python", "sodecl.cpp")).read()
+    src = src.replace('#include "sodecl.hpp"', '#include "sodeclmgr.hpp"')
+    cpp = tmp_path / "sodecl.cpp"
+    cpp.write_text(src)
+    out = tmp_path / ("sodecl_interface" + sysconfig.get_config_var("EXT_SUFFIX"))
+    pkg = os.path.join(ROOT, "sodecl_amd")
+    cmd = ["g++", "-O0", "-shared", "-fPIC", "-std=c++17", f"-I{pkg}/csrc", f"-I{ROOT}/include",
+           f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}", str(cpp),
+           os.path.join(pkg, "csrc", "sodeclmgr.cpp"), f"-L{pkg}", "-lsodecl_amd", f"-Wl,-rpath,{pkg}", "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert out.exists()
