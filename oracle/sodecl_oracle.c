@@ -576,3 +576,46 @@ long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_co
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Check of the division identity the MI355X code and the bundled plugins   */
+/* use for divisions by constants and by orbit parameters: with rd = RN(1/d)*/
+/* q0 = RN(a rd), r = a - q0 d (exact, one fma), q1 = RN(q0 + r rd) equals   */
+/* RN(a / d) (Markstein); copysign/inf handling as in the kernels.  Counts   */
+/* mismatches over n random normal dividends plus special values.           */
+/* ------------------------------------------------------------------------ */
+static double orc_markstein(double a, double d, double rd)
+{
+    const double q0 = a * rd;
+    const double r = fma(-q0, d, a);
+    const double q1 = fma(r, rd, q0);
+    return isfinite(q0) ? copysign(q1, q0) : q0;
+}
+
+long orc_markstein_mismatches(double d, long n, uint64_t seed)
+{
+    uint64_t s0 = seed ^ 0x9E3779B97F4A7C15ULL, s1 = seed * 0xBF58476D1CE4E5B9ULL + 1;
+    const double rd = 1.0 / d;
+    long bad = 0;
+    const double special[] = {0.0, -0.0, INFINITY, -INFINITY, 1.0, -1.0, d, -d, 3.0 * d, 1e300, -1e300, 1e-300};
+    for (size_t i = 0; i < sizeof special / sizeof special[0]; i++) {
+        double q = special[i] / d, m = orc_markstein(special[i], d, rd);
+        if (memcmp(&q, &m, 8) != 0) bad++;
+    }
+    double nq = NAN / d, nm = orc_markstein(NAN, d, rd);
+    if (!(isnan(nq) && isnan(nm))) bad++;
+    for (long i = 0; i < n; i++) {
+        uint64_t x = s0, y = s1;   /* xorshift128+ */
+        s0 = y;
+        x ^= x << 23;
+        s1 = x ^ y ^ (x >> 17) ^ (y >> 26);
+        uint64_t m = s1 + y;
+        uint64_t e = ((m >> 52) % 1600) + 200;   /* exponents well inside the normal range */
+        uint64_t bits = (m & 0x800FFFFFFFFFFFFFULL) | (e << 52);
+        double a;
+        memcpy(&a, &bits, 8);
+        double q = a / d, mm = orc_markstein(a, d, rd);
+        if (memcmp(&q, &mm, 8) != 0) bad++;
+    }
+    return bad;
+}

@@ -37,6 +37,22 @@
 
 typedef unsigned long long sodecl_u64;
 
+// a / d for a divisor d whose correctly rounded reciprocal rd = 1/d is known
+// (a constant, or hoisted out of the step loop): q0 = a*rd is within one ulp,
+// the remainder a - q0*d is exact in one fused multiply-add, and one more
+// corrects q0 to the correctly rounded quotient (Markstein's theorem), the
+// same bits as IEEE a / d outside the subnormal range.  The correction never
+// changes the sign, so copying q0's sign keeps -0; infinities and NaN pass
+// through.  Three full-rate FP64 operations instead of the quarter-rate
+// reciprocal and scale/fixup sequence of a general division.
+SODECL_INLINE double sodecl_div(double a, double d, double rd)
+{
+    const double q0 = a * rd;
+    const double r = fma(-q0, d, a);
+    const double q1 = fma(r, rd, q0);
+    return __builtin_isfinite(q0) ? __builtin_copysign(q1, q0) : q0;
+}
+
 // ---------------------------------------------------------------------------
 // Threefry-2x64, 20 rounds (Salmon, Moraes, Dror, Shaw, "Parallel Random
 // Numbers: As Easy as 1, 2, 3", SC'11): Threefish-style add/rotate/xor rounds
@@ -240,7 +256,7 @@ SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
     }
     sode_system(t + _m_dt_, x, k, p);
     SODECL_UNROLL
-    for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + (yout[i] + k[i]) * _m_dt_ / 6.00;
+    for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + sodecl_div((yout[i] + k[i]) * _m_dt_, 6.00, 1.0 / 6.00);
     return 0;
 }
 #elif SODECL_IS_IMPLICIT
@@ -437,7 +453,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
     for (int c = 0; c < SODECL_CPL; ++c) {
         const int comp = lane + c * SODECL_G;
         const double f = comp < _numeq_ ? sode_system_i(comp, t + _m_dt_, xs[buf][slot], pp) : 0.0;
-        y[c] = y[c] + (acc[c] + f) * _m_dt_ / 6.00;
+        y[c] = y[c] + sodecl_div((acc[c] + f) * _m_dt_, 6.00, 1.0 / 6.00);
     }
     buf ^= 1;
 }

@@ -29,6 +29,19 @@
 #define OCU_T2 0.012
 #define OCU_TN 25.00
 
+/* a / d given rd = 1/d correctly rounded (d a constant or an orbit
+ * parameter, so rd is computed once): q0 = a*rd, exact remainder by one
+ * fused multiply-add, one correction (Markstein) -- the bits of IEEE a / d
+ * outside the subnormal range, without a general division per call.  The
+ * correction cannot change the sign (copysign keeps -0); inf/NaN pass. */
+double ocu_div(double a, double d, double rd)
+{
+    const double q0 = a * rd;
+    const double r = fma(-q0, d, a);
+    const double q1 = fma(r, rd, q0);
+    return isfinite(q0) ? copysign(q1, q0) : q0;
+}
+
 /* Burst-neuron firing curve: saturating exponential for x > 0, a scaled
  * x*exp(x/b) ramp otherwise.  The exponent is selected before the single
  * exp() call, so lanes of a wavefront that take different sides still
@@ -38,7 +51,11 @@
 double ocu_burst(double x, double a, double b, double at, double bt)
 {
     const int pos = x > 0;
-    const double e = exp((pos ? -x : x) / (pos ? bt : b));
+    /* both candidate exponents through the parameters' reciprocals, which
+     * depend on the orbit only and so are computed once per run */
+    const double up = ocu_div(-x, bt, 1.0 / bt);
+    const double dn = ocu_div(x, b, 1.0 / b);
+    const double e = exp(pos ? up : dn);
     return pos ? at * (1.00 - e) : -(a / b) * x * e;
 }
 
@@ -58,7 +75,7 @@ void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_nu
 
     yout[0] = y[1];
     yout[1] = -(rate_sum * y[1]) - (rate_prod * y[0]) + (rate_prod * y[2]) + rate_sum * drive;
-    yout[2] = -(y[2] / OCU_TN) + drive;
+    yout[2] = -ocu_div(y[2], OCU_TN, 1.0 / OCU_TN) + drive;
     yout[3] = (1.00 / p[2]) * (-y[3] - (p[3] * y[3] * y[4] * y[4]) + ocu_burst(y[5], p[0], p[1], p[4], p[5]));
     yout[4] = (1.00 / p[2]) * (-y[4] - (p[3] * y[4] * y[3] * y[3]) + ocu_burst(-y[5], p[0], p[1], p[4], p[5]));
     yout[5] = -drive;

@@ -220,3 +220,16 @@ def test_plugin_matches_reference_example(solver):
     a = RefKernels("oculomotor", solver).run(y0, p, 1e-6, 2e-3, 40)
     b = RefKernels("refoculomotor", solver).run(y0, p, 1e-6, 2e-3, 40)
     np.testing.assert_array_equal(a["out"], b["out"])
+
+
+@pytest.mark.parametrize("d", [6.0, 25.0, 0.15, 9.0, 1.5, 0.0045 * 1.7])
+def test_markstein_division_is_correctly_rounded(ocu, d):
+    """The division the kernels use for constant and per-orbit divisors
+    (q0 = a/d via the reciprocal, one exact-remainder correction) gives the
+    bits of IEEE a / d: random dividends over the normal range plus zeros,
+    infinities, NaN and exact quotients."""
+    import ctypes
+    f = ocu.lib.orc_markstein_mismatches
+    f.restype = ctypes.c_long
+    f.argtypes = [ctypes.c_double, ctypes.c_long, ctypes.c_uint64]
+    assert f(d, 2_000_000, 12345) == 0
