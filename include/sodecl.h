@@ -70,8 +70,10 @@ typedef struct sodecl_options {
     int newton_max_iter;     /* 0 = 1000 (src/kernels/ie.cl:69, im.cl:65) */
     int segments_per_launch; /* ksteps segments per kernel launch; 0 = as many as the
                                 trajectory buffer budget allows */
-    int use_graph;           /* replay the launch sequence of a run from a captured hipGraph */
-    int autotune;            /* time the candidate schedules once per code object and keep the fastest */
+    int use_graph;           /* device runs of several launches (segments_per_launch set): record the
+                                launch sequence once as a hipGraph and replay it */
+    int autotune;            /* with schedule AUTO: time the candidate schedules, group sizes and block
+                                sizes on one segment at setup and keep the fastest (cached on disk) */
     int fp_contract;         /* 0 = fused multiply-add contraction allowed (default), 1 = off */
     int verbose;             /* print "Compute results runtime" like the reference */
     size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
@@ -208,6 +210,10 @@ double sodecl_last_kernel_ms(const sodecl_ctx* ctx);
 
 /* Schedule, group size and block size the last run used. */
 int sodecl_get_launch_config(const sodecl_ctx* ctx, int* schedule, int* group_size, int* block_size);
+
+/* What the autotuner measured (candidate: time of one ksteps segment), or
+ * "cached" when the choice came from its on-disk cache; empty without it. */
+const char* sodecl_autotune_report(const sodecl_ctx* ctx);
 
 /* Newton updates per orbit accumulated over the runs (implicit solvers);
  * counts[orbits]. */

@@ -296,6 +296,9 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
     double noise[SODECL_NNOI_ALLOC];
 #endif
     long long nu = 0;
+    // checkpoint k is the state before global step k * ckpt_every
+    long long ck = ckpt_y ? (gstep0 + ckpt_every - 1) / ckpt_every : 0;   // next checkpoint index
+    long long ck_step = ck * ckpt_every;
     long long g = gstep0;
     for (int s = 0; s < nseg; ++s) {
         if (traj) {
@@ -304,8 +307,9 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
             for (int m = 0; m < _numeq_; ++m) o[(size_t)m * orbits] = y[m];
         }
         for (int it = 0; it < _numsteps_; ++it, ++g) {
-            if (ckpt_y && g % ckpt_every == 0) {
-                const long long k = g / ckpt_every;
+            if (ckpt_y && g == ck_step) {
+                const long long k = ck++;
+                ck_step += ckpt_every;
                 SODECL_UNROLL
                 for (int m = 0; m < _numeq_; ++m) ckpt_y[(size_t)k * slab + (size_t)m * orbits + i] = y[m];
                 ckpt_t[(size_t)k * orbits + i] = t;
@@ -499,6 +503,9 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
 #endif
     int buf = 0;
     long long g = gstep0;
+    // checkpoint k is the state before global step k * ckpt_every
+    long long ck = ckpt_y ? (gstep0 + ckpt_every - 1) / ckpt_every : 0;
+    long long ck_step = ck * ckpt_every;
 
     for (int s = 0; s < nseg; ++s) {
         if (traj && active) {
@@ -509,14 +516,17 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
             }
         }
         for (int it = 0; it < _numsteps_; ++it, ++g) {
-            if (ckpt_y && active && g % ckpt_every == 0) {
-                const long long k = g / ckpt_every;
+            if (ckpt_y && g == ck_step) {
+                const long long k = ck++;
+                ck_step += ckpt_every;
+                if (active) {
 #pragma unroll
-                for (int c = 0; c < SODECL_CPL; ++c) {
-                    const int comp = lane + c * SODECL_G;
-                    if (comp < _numeq_) ckpt_y[(size_t)k * slab + (size_t)comp * orbits + orbit] = y[c];
+                    for (int c = 0; c < SODECL_CPL; ++c) {
+                        const int comp = lane + c * SODECL_G;
+                        if (comp < _numeq_) ckpt_y[(size_t)k * slab + (size_t)comp * orbits + orbit] = y[c];
+                    }
+                    if (lane == 0) ckpt_t[(size_t)k * orbits + orbit] = t;
                 }
-                if (lane == 0) ckpt_t[(size_t)k * orbits + orbit] = t;
             }
 #if SODECL_IS_SDE
             // Noise: Threefry call k of this step goes to lane k of the group;

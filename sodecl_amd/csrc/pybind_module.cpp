@@ -229,6 +229,7 @@ public:
         return n;
     }
     std::string program_source() { return sodecl_program_source(ctx_); }
+    std::string autotune_report() const { return sodecl_autotune_report(ctx_); }
     void set_options(py::dict options)
     {
         apply_options(opts_, options);
@@ -360,6 +361,7 @@ PYBIND11_MODULE(sodecl_interface, m)
         .def("last_runtime", &Session::last_runtime)
         .def("compile_only", &Session::compile_only, py::arg("arch") = "gfx950")
         .def("program_source", &Session::program_source)
+        .def("autotune_report", &Session::autotune_report)
         .def("set_options", &Session::set_options)
         .def("handle", &Session::handle)
         .def_property_readonly("nsnap", &Session::nsnap)

@@ -13,12 +13,14 @@
 //   * the program is compiled once per configuration and cached.
 // The reference's launch design is kept as SODECL_SCHEDULE_PLAIN, the
 // baseline the fast schedules are measured against.
+#include <dlfcn.h>
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -193,6 +195,9 @@ int default_group(int neq)
 LaunchCfg choose_config(const sodecl_ctx* c)
 {
     LaunchCfg cfg{1, 64, 128};
+    // a timed choice made by the autotuner overrides the heuristic
+    if (c->opts.schedule == SODECL_SCHEDULE_AUTO && c->tuned_schedule)
+        return LaunchCfg{c->tuned_schedule, c->tuned_group, c->tuned_block};
     int sched = c->opts.schedule;
     if (sched == SODECL_SCHEDULE_AUTO) sched = (group_possible(c) && c->neq >= 8) ? 2 : 1;
     if (sched == SODECL_SCHEDULE_GROUP && !group_possible(c)) sched = 1;
@@ -464,6 +469,116 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
     return 1;
 }
 
+// Candidate launch configurations for the autotuner: the lane schedule at
+// three block sizes and, when the plugin allows it, the group schedule at
+// every power-of-two group size from the system size's own power of two down
+// to a quarter of it, at two block sizes.
+std::vector<LaunchCfg> tune_candidates(const sodecl_ctx* c)
+{
+    std::vector<LaunchCfg> out;
+    const bool lane_ok = c->neq <= 64;   // larger states spill to scratch memory
+    if (lane_ok)
+        for (int b : {64, 128, 256}) out.push_back(LaunchCfg{1, 1, b});
+    if (group_possible(c)) {
+        int g0 = default_group(c->neq);
+        for (int g = g0; g >= std::max(2, g0 / 4); g >>= 1)
+            for (int b : {64, 256})
+                if (b >= g) out.push_back(LaunchCfg{2, g, b});
+    }
+    if (out.empty()) out.push_back(LaunchCfg{1, 1, 128});
+    return out;
+}
+
+std::string tune_cache_path(const sodecl_ctx* c)
+{
+    // keyed by everything that selects the programs (text of one candidate)
+    ProgramKey k = sodecl_amd::make_key(c, 0, 0, 0, std::string());
+    std::string text = assemble_program(c->system_source, k, c->features) + device_arch(c->device);
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char ch : text) {
+        h ^= ch;
+        h *= 1099511628211ull;
+    }
+    char name[64];
+    std::snprintf(name, sizeof name, "tune_%016llx.txt", (unsigned long long)h);
+    const char* dir = std::getenv("SODECL_AMD_CACHE_DIR");
+    std::string d = dir ? dir : std::string();
+    if (d.empty()) {
+        // next to the code-object cache
+        Dl_info info;
+        if (dladdr((void*)&tune_candidates, &info) && info.dli_fname) {
+            std::string p(info.dli_fname);
+            d = p.substr(0, p.rfind('/')) + "/_jit_cache";
+        } else {
+            d = "/tmp/sodecl_amd_jit_cache";
+        }
+    }
+    return d + "/" + name;
+}
+
+// Times every candidate on one launch of one ksteps segment over all orbits
+// (on scratch copies of the state, so the context's state is untouched) and
+// keeps the fastest.  The choice is cached on disk per configuration.
+int autotune(sodecl_ctx* c)
+{
+    const std::string path = tune_cache_path(c);
+    {
+        std::ifstream f(path);
+        int s = 0, g = 0, b = 0;
+        if (f >> s >> g >> b && (s == 1 || s == 2) && b > 0) {
+            c->tuned_schedule = s;
+            c->tuned_group = g;
+            c->tuned_block = b;
+            c->tune_report = "cached";
+            return 1;
+        }
+    }
+    const size_t n = (size_t)c->orbits;
+    double *t = nullptr, *y = nullptr, *cnt = nullptr;
+    if (!hip_ok(c, hipMalloc(&t, n * sizeof(double)), "hipMalloc(tune)")) return 0;
+    if (!hip_ok(c, hipMalloc(&y, n * c->neq * sizeof(double)), "hipMalloc(tune)")) return 0;
+    if (c->d_counters && !hip_ok(c, hipMalloc(&cnt, n * sizeof(double)), "hipMalloc(tune)")) return 0;
+    std::ostringstream rep;
+    float best = 1e30f;
+    LaunchCfg best_cfg{0, 0, 0};
+    for (const LaunchCfg& cfg : tune_candidates(c)) {
+        if (!get_program(c, cfg.schedule, cfg.group, cfg.block)) {
+            c->last_error.clear();
+            continue;
+        }
+        float ms_best = 1e30f;
+        for (int rep_i = 0; rep_i < 2; ++rep_i) {   // the first launch also loads the code object
+            hipMemcpyAsync(t, c->d_t, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+            hipMemcpyAsync(y, c->d_y, n * c->neq * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+            if (cnt) hipMemcpyAsync(cnt, c->d_counters, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+            hipEventRecord(c->ev_start, c->stream);
+            if (!enqueue(c, cfg, t, y, c->d_params, cnt, nullptr, nullptr, 0, 1, 0, c->stream)) break;
+            hipEventRecord(c->ev_stop, c->stream);
+            if (!hip_ok(c, hipEventSynchronize(c->ev_stop), "autotune")) break;
+            float ms = 0;
+            hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+            ms_best = std::min(ms_best, ms);
+        }
+        rep << (cfg.schedule == 2 ? "group" : "lane") << " G=" << cfg.group << " B=" << cfg.block << ": "
+            << ms_best << " ms\n";
+        if (ms_best < best) {
+            best = ms_best;
+            best_cfg = cfg;
+        }
+    }
+    hipFree(t);
+    hipFree(y);
+    hipFree(cnt);
+    if (!best_cfg.schedule) return fail(c, "autotune: no candidate ran");
+    c->tuned_schedule = best_cfg.schedule;
+    c->tuned_group = best_cfg.group;
+    c->tuned_block = best_cfg.block;
+    c->tune_report = rep.str();
+    std::ofstream f(path);
+    if (f) f << best_cfg.schedule << " " << best_cfg.group << " " << best_cfg.block << "\n";
+    return 1;
+}
+
 }  // namespace
 
 namespace sodecl_amd {
@@ -488,14 +603,48 @@ int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params,
         if (traj) return enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, 0, 0, 1, s);
         return 1;
     }
-    for (long s0 = 0; s0 < launches; s0 += per) {
-        int n = (int)std::min(per, launches - s0);
-        bool last = s0 + n >= launches;
-        if (!enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, (int)s0, n, last ? 1 : 0, s, ckpt_t, ckpt_y,
-                     ckpt_every, (long long)s0 * c->ksteps))
+    auto enqueue_all = [&](hipStream_t st) -> int {
+        for (long s0 = 0; s0 < launches; s0 += per) {
+            int n = (int)std::min(per, launches - s0);
+            bool last = s0 + n >= launches;
+            if (!enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, (int)s0, n, last ? 1 : 0, st, ckpt_t,
+                         ckpt_y, ckpt_every, (long long)s0 * c->ksteps))
+                return 0;
+        }
+        return 1;
+    };
+    if (!c->opts.use_graph || launches <= per) return enqueue_all(s);
+    // Several launches (segments_per_launch set): record the sequence once
+    // as a hipGraph on the context's stream and replay it on the caller's
+    // stream; the recording is reused while pointers and shape stay the same.
+    std::ostringstream key;
+    key << (void*)t0 << (void*)y0 << (const void*)params << (void*)counters << (void*)traj << (void*)ckpt_t
+        << (void*)ckpt_y << ":" << ckpt_every << ":" << launches << ":" << per << ":" << cfg.schedule << ":"
+        << cfg.group << ":" << cfg.block;
+    if (c->graph_key != key.str() || !c->graph_exec) {
+        if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+        if (c->graph) hipGraphDestroy(c->graph);
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        c->graph_key.clear();
+        // compile before recording: no JIT inside a capture
+        if (!get_program(c, cfg.schedule, cfg.group, cfg.block)) return 0;
+        if (!hip_ok(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
             return 0;
+        int ok = enqueue_all(c->stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (!ok) {
+            if (g) hipGraphDestroy(g);
+            return 0;
+        }
+        if (!hip_ok(c, e, "hipStreamEndCapture")) return 0;
+        c->graph = g;
+        if (!hip_ok(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate"))
+            return 0;
+        c->graph_key = key.str();
     }
-    return 1;
+    return hip_ok(c, hipGraphLaunch(c->graph_exec, s), "hipGraphLaunch") ? 1 : 0;
 }
 
 int launch_config(const sodecl_ctx* c, int* schedule, int* group, int* block)
@@ -675,6 +824,8 @@ int sodecl_setup(sodecl_ctx* c)
                     "hipMemcpy(counters)"))
             return 0;
     }
+    // the autotuner times the candidate schedules once per configuration
+    if (c->opts.schedule == SODECL_SCHEDULE_AUTO && c->opts.autotune && !autotune(c)) return 0;
     // compile the program the first run will use (JIT outside the timed loop)
     LaunchCfg cfg = choose_config(c);
     if (!get_program(c, cfg.schedule, cfg.group, cfg.block)) return 0;
@@ -776,6 +927,8 @@ size_t sodecl_compile_only(sodecl_ctx* c, const char* arch)
 }
 
 const char* sodecl_program_source(sodecl_ctx* c) { return c ? c->last_program.c_str() : ""; }
+
+const char* sodecl_autotune_report(const sodecl_ctx* c) { return c ? c->tune_report.c_str() : ""; }
 
 const char* sodecl_last_error(const sodecl_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
 

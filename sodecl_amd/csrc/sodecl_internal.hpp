@@ -110,7 +110,9 @@ struct sodecl_ctx {
     // programs: one per (schedule, group, block)
     std::map<std::string, std::shared_ptr<sodecl_amd::Program>> programs;
     int used_schedule = 0, used_group = 0, used_block = 0;
-    bool tuned = false;
+    // autotuner result (schedule, group, block); tuned_schedule 0 = none
+    int tuned_schedule = 0, tuned_group = 0, tuned_block = 0;
+    std::string tune_report;
 
     // output (sodeclmgr::m_output)
     std::vector<double> output;

@@ -1,0 +1,70 @@
+"""GPU tests of the launch machinery: the autotuner and hipGraph replay.
+Both must leave results unchanged: the tuned configuration is checked
+against the oracle at 1e-10, graph replays against the eager launches bit
+for bit."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from sodecl_amd import configs
+from oracle.oracle import Oracle
+from tests.parity import assert_traj_close
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def test_autotune_choice_is_timed_cached_and_correct(tmp_path, monkeypatch):
+    monkeypatch.setenv("SODECL_AMD_CACHE_DIR", str(tmp_path))
+    cfg = configs.kuramoto_euler(orbits=777, n=20, seed=9)
+    cfg["tspan"] = 1.0
+    s = configs.session_for(cfg, autotune=1)
+    s.setup(cfg["y0"], cfg["params"])
+    report = s.autotune_report()
+    assert "group" in report and "lane" in report, report
+    out = s.run().reshape(s.nsnap, 20, 777)
+    ref = Oracle("kuramoto").run("euler", cfg["y0"], cfg["params"], cfg["dt"], cfg["tspan"], cfg["ksteps"])["out"]
+    assert_traj_close(out, ref, what=f"tuned {s.launch_config()}")
+    s2 = configs.session_for(cfg, autotune=1)
+    s2.setup(cfg["y0"], cfg["params"])
+    assert s2.autotune_report() == "cached"
+    assert any(f.startswith("tune_") for f in os.listdir(tmp_path))
+
+
+def test_graph_replay_matches_eager_launches():
+    cfg = configs.oculomotor_rk4(orbits=300, seed=3, dt=2e-6, tspan=0.004, ksteps=100)
+    nsteps_launch = 1   # one launch per ksteps segment: 20 launches recorded in the graph
+
+    def device_run(session, y_init, p, t, traj, stream):
+        y = y_init.clone()
+        t.zero_()
+        session.run_device(t.data_ptr(), y.data_ptr(), p.data_ptr(), 0, traj.data_ptr(), stream)
+        return y
+
+    y_init = torch.tensor(cfg["y0"], device=DEV)
+    p = torch.tensor(cfg["params"], device=DEV)
+    eager = configs.session_for(cfg, schedule="lane", segments_per_launch=nsteps_launch)
+    eager.setup(cfg["y0"], cfg["params"])
+    t = torch.zeros(300, dtype=torch.float64, device=DEV)
+    traj_e = torch.empty((eager.nsnap, 6, 300), dtype=torch.float64, device=DEV)
+    ye = device_run(eager, y_init, p, t, traj_e, 0)
+    torch.cuda.synchronize()
+
+    graph = configs.session_for(cfg, schedule="lane", segments_per_launch=nsteps_launch, use_graph=1)
+    graph.setup(cfg["y0"], cfg["params"])
+    y = torch.empty_like(y_init)
+    tg = torch.zeros(300, dtype=torch.float64, device=DEV)
+    traj_g = torch.empty_like(traj_e)
+    for rep in range(4):
+        y.copy_(y_init)                      # eager work between replays
+        tg.zero_()
+        traj_g.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.run_device(tg.data_ptr(), y.data_ptr(), p.data_ptr(), 0, traj_g.data_ptr(), 0)
+        torch.cuda.synchronize()
+        assert torch.equal(traj_g, traj_e), f"replay {rep}"
+        assert torch.equal(y, ye), f"replay {rep}"
+    ref = Oracle("oculomotor").run("rk4", cfg["y0"], cfg["params"], cfg["dt"], cfg["tspan"], cfg["ksteps"])["out"]
+    assert_traj_close(traj_g.cpu().numpy(), ref, what="graph")
