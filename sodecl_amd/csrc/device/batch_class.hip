@@ -18,19 +18,43 @@
 // of one sweep changes only the rounding.
 
 #define SODECL_BR ((_numeq_ + SODECL_BG - 1) / SODECL_BG)   // rows per partition == components per lane
+// LDS index of row i in the solve's arrays: one pad slot per partition, so
+// the 64 lanes reading row j of their own partitions (stride R + 1 doubles)
+// spread over the LDS banks instead of colliding on a stride of R.
+#if SODECL_BR >= 2
+#define SODECL_TI(i) ((i) + (i) / SODECL_BR)
+#define SODECL_TLEN (_numeq_ + SODECL_BG)
+#else
+#define SODECL_TI(i) (i)
+#define SODECL_TLEN (_numeq_)
+#endif
 
 // Coefficients of row i of the reference's tridiagonal matrix.
-SODECL_INLINE double sodecl_tri_a(const double* up, int i) { return (i >= 1 && i < _numeq_ - 2) ? up[i] : 0.0; }
-SODECL_INLINE double sodecl_tri_c(const double* lo, int i) { return (i < _numeq_ - 1) ? lo[i] : 0.0; }
+SODECL_INLINE double sodecl_tri_a(const double* up, int i)
+{
+    return (i >= 1 && i < _numeq_ - 2) ? up[SODECL_TI(i)] : 0.0;
+}
+SODECL_INLINE double sodecl_tri_c(const double* lo, int i) { return (i < _numeq_ - 1) ? lo[SODECL_TI(i)] : 0.0; }
 
-SODECL_INLINE void sodecl_pcr_neighbour(double v, int lane, int s, bool up, double identity, double* out)
+// 1/b from the hardware reciprocal and two Newton-Raphson steps (within an
+// ulp); this solve is not the reference's sweep, so only rounding changes.
+SODECL_INLINE double sodecl_rcp(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return r;
+}
+
+SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up, double identity)
 {
     const double w = up ? __shfl_up(v, (unsigned)s, SODECL_BG) : __shfl_down(v, (unsigned)s, SODECL_BG);
     const bool inside = up ? lane - s >= 0 : lane + s < SODECL_BG;
-    *out = inside ? w : identity;
+    return inside ? w : identity;
 }
 
-// Solves M x = v for the matrix above; lo, di, up, v, x in LDS; all 64 lanes.
+// Solves M x = v for the matrix above; lo, di, up, v, x in LDS (padded
+// layout); all 64 lanes.
 SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const double* up, const double* v, double* x,
                                     int lane)
 {
@@ -39,7 +63,7 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     const bool active = r0 < _numeq_;
     const int r1 = active ? min(_numeq_, r0 + R) - 1 : r0;
     const int nin = active ? r1 - r0 : 0;   // interior rows r0 .. r1-1
-    double al[SODECL_BR], be[SODECL_BR], ga[SODECL_BR];
+    double ia[SODECL_BR], be[SODECL_BR], ga[SODECL_BR];   // ia = 1/alpha
     double e[SODECL_BR], f[SODECL_BR], g[SODECL_BR];
     // forward elimination through the interior, tracking the coupling to
     // x_L = last unknown of the previous partition
@@ -47,16 +71,18 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     for (int j = 0; j < SODECL_BR - 1; ++j) {
         if (j < nin) {
             const int i = r0 + j;
+            double al;
             if (j == 0) {
-                al[0] = di[i];
-                be[0] = v[i];
+                al = di[SODECL_TI(i)];
+                be[0] = v[SODECL_TI(i)];
                 ga[0] = sodecl_tri_a(up, i);
             } else {
-                const double m = sodecl_tri_a(up, i) / al[j - 1];
-                al[j] = di[i] - m * sodecl_tri_c(lo, i - 1);
-                be[j] = v[i] - m * be[j - 1];
+                const double m = sodecl_tri_a(up, i) * ia[j - 1];
+                al = di[SODECL_TI(i)] - m * sodecl_tri_c(lo, i - 1);
+                be[j] = v[SODECL_TI(i)] - m * be[j - 1];
                 ga[j] = -m * ga[j - 1];
             }
+            ia[j] = sodecl_rcp(al);
         }
     }
     // backward: interior x_i = e_i + f_i x_L + g_i x_R, x_R = own last unknown
@@ -64,16 +90,15 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     for (int j = SODECL_BR - 2; j >= 0; --j) {
         if (j < nin) {
             const int i = r0 + j;
-            const double inv = 1.0 / al[j];
             const double ci = sodecl_tri_c(lo, i);
             if (j == nin - 1) {
-                e[j] = be[j] * inv;
-                f[j] = -ga[j] * inv;
-                g[j] = -ci * inv;
+                e[j] = be[j] * ia[j];
+                f[j] = -ga[j] * ia[j];
+                g[j] = -ci * ia[j];
             } else {
-                e[j] = (be[j] - ci * e[j + 1]) * inv;
-                f[j] = (-ga[j] - ci * f[j + 1]) * inv;
-                g[j] = (-ci * g[j + 1]) * inv;
+                e[j] = (be[j] - ci * e[j + 1]) * ia[j];
+                f[j] = (-ga[j] - ci * f[j + 1]) * ia[j];
+                g[j] = (-ci * g[j + 1]) * ia[j];
             }
         }
     }
@@ -95,24 +120,25 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
         const double cr = sodecl_tri_c(lo, r1);
         const bool has_next = r1 + 1 < _numeq_;
         A = ar * f_last;
-        B = di[r1] + ar * g_last + (has_next ? cr * f_nx : 0.0);
+        B = di[SODECL_TI(r1)] + ar * g_last + (has_next ? cr * f_nx : 0.0);
         C = has_next ? cr * g_nx : 0.0;
-        D = v[r1] - ar * e_last - (has_next ? cr * e_nx : 0.0);
+        D = v[SODECL_TI(r1)] - ar * e_last - (has_next ? cr * e_nx : 0.0);
     }
-    // parallel cyclic reduction over the 64 reduced rows
+    // parallel cyclic reduction over the 64 reduced rows; each lane shares
+    // its pivot's reciprocal so one reciprocal per lane and level suffices
 #pragma unroll
     for (int s = 1; s < SODECL_BG; s <<= 1) {
-        double Am, Bm, Cm, Dm, Ap, Bp, Cp, Dp;
-        sodecl_pcr_neighbour(A, lane, s, true, 0.0, &Am);
-        sodecl_pcr_neighbour(B, lane, s, true, 1.0, &Bm);
-        sodecl_pcr_neighbour(C, lane, s, true, 0.0, &Cm);
-        sodecl_pcr_neighbour(D, lane, s, true, 0.0, &Dm);
-        sodecl_pcr_neighbour(A, lane, s, false, 0.0, &Ap);
-        sodecl_pcr_neighbour(B, lane, s, false, 1.0, &Bp);
-        sodecl_pcr_neighbour(C, lane, s, false, 0.0, &Cp);
-        sodecl_pcr_neighbour(D, lane, s, false, 0.0, &Dp);
-        const double k1 = A / Bm;
-        const double k2 = C / Bp;
+        const double iB = sodecl_rcp(B);
+        const double Am = sodecl_pcr_neighbour(A, lane, s, true, 0.0);
+        const double iBm = sodecl_pcr_neighbour(iB, lane, s, true, 1.0);
+        const double Cm = sodecl_pcr_neighbour(C, lane, s, true, 0.0);
+        const double Dm = sodecl_pcr_neighbour(D, lane, s, true, 0.0);
+        const double Ap = sodecl_pcr_neighbour(A, lane, s, false, 0.0);
+        const double iBp = sodecl_pcr_neighbour(iB, lane, s, false, 1.0);
+        const double Cp = sodecl_pcr_neighbour(C, lane, s, false, 0.0);
+        const double Dp = sodecl_pcr_neighbour(D, lane, s, false, 0.0);
+        const double k1 = A * iBm;
+        const double k2 = C * iBp;
         const double nA = -k1 * Am;
         const double nC = -k2 * Cp;
         const double nB = B - k1 * Cm - k2 * Ap;
@@ -122,14 +148,13 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
         C = nC;
         D = nD;
     }
-    const double X = D / B;
-    double Xm;
-    sodecl_pcr_neighbour(X, lane, 1, true, 0.0, &Xm);
+    const double X = D * sodecl_rcp(B);
+    const double Xm = sodecl_pcr_neighbour(X, lane, 1, true, 0.0);
     if (active) {
 #pragma unroll
         for (int j = 0; j < SODECL_BR - 1; ++j)
-            if (j < nin) x[r0 + j] = e[j] + f[j] * Xm + g[j] * X;
-        x[r1] = X;
+            if (j < nin) x[SODECL_TI(r0 + j)] = e[j] + f[j] * Xm + g[j] * X;
+        x[SODECL_TI(r1)] = X;
     }
 }
 
@@ -143,13 +168,13 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
                                         double* __restrict__ out, int steps, double* lds)
 {
     const int lane = threadIdx.x;
-    double* const sh_g = lds;                  // evaluation point (g, or (y+g)/2)
-    double* const sh_r = lds + _numeq_;        // residual -> right-hand side of the solve
-    double* const sh_lo = lds + 2 * _numeq_;
-    double* const sh_di = lds + 3 * _numeq_;
-    double* const sh_up = lds + 4 * _numeq_;
-    double* const sh_x = lds + 5 * _numeq_;
-    double* const sh_p = lds + 6 * _numeq_;
+    double* const sh_g = lds;                          // evaluation point (g, or (y+g)/2), plain layout
+    double* const sh_r = lds + _numeq_;                // residual -> right-hand side (padded layout)
+    double* const sh_lo = sh_r + SODECL_TLEN;
+    double* const sh_di = sh_lo + SODECL_TLEN;
+    double* const sh_up = sh_di + SODECL_TLEN;
+    double* const sh_x = sh_up + SODECL_TLEN;
+    double* const sh_p = sh_x + SODECL_TLEN;
     for (int k = lane; k < _numpar_; k += SODECL_BG) sh_p[k] = prm[k];
     double y[SODECL_BR], gg[SODECL_BR];
 #pragma unroll
@@ -208,10 +233,10 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
                 if (comp < _numeq_) {
                     double lo, di, up;
                     calc_jacobian_band_i(comp, sh_g, sh_p, &lo, &di, &up);
-                    sh_lo[comp] = lo * scale * _m_dt_;
-                    sh_up[comp] = up * scale * _m_dt_;
-                    sh_di[comp] = di * scale * _m_dt_ + 1;
-                    sh_r[comp] = r[c];
+                    sh_lo[SODECL_TI(comp)] = lo * scale * _m_dt_;
+                    sh_up[SODECL_TI(comp)] = up * scale * _m_dt_;
+                    sh_di[SODECL_TI(comp)] = di * scale * _m_dt_ + 1;
+                    sh_r[SODECL_TI(comp)] = r[c];
                 }
             }
             __syncthreads();
@@ -220,7 +245,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
-                if (comp < _numeq_) gg[c] = gg[c] - sh_x[comp];
+                if (comp < _numeq_) gg[c] = gg[c] - sh_x[SODECL_TI(comp)];
             }
             ++updates;
         }
@@ -240,3 +265,5 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 }
 
 #undef SODECL_BR
+#undef SODECL_TI
+#undef SODECL_TLEN
