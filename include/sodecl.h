@@ -77,6 +77,8 @@ typedef struct sodecl_options {
     int fp_contract;         /* 0 = fused multiply-add contraction allowed (default), 1 = off */
     int verbose;             /* print "Compute results runtime" like the reference */
     size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
+    int disable_pairs;       /* 1: under the group schedule evaluate every component with sode_system_i even
+                                when the plugin has the pairwise form (sode_pair, sode_system_pairsum) */
 } sodecl_options;
 
 typedef struct sodecl_ctx sodecl_ctx;
@@ -208,7 +210,7 @@ const char* sodecl_batch_last_error(const sodecl_batch* batch);
 double sodecl_last_runtime(const sodecl_ctx* ctx);
 double sodecl_last_kernel_ms(const sodecl_ctx* ctx);
 
-/* Schedule, group size and block size the last run used. */
+/* Schedule, group size and block size the last run (or sodecl_compile_only) used. */
 int sodecl_get_launch_config(const sodecl_ctx* ctx, int* schedule, int* group_size, int* block_size);
 
 /* What the autotuner measured (candidate: time of one ksteps segment), or

@@ -62,6 +62,12 @@ sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict
     const int ob = active ? orbit : 0;   // inactive groups read orbit 0's inputs and store no result
     double* const pp = sh_p[slot];
     double* const gp = sh_gp[slot];
+#if SODECL_USE_PAIRS
+    __shared__ double sh_S[SODECL_OPB][_numeq_ * SODECL_SSTRIDE];
+    double* const S = sh_S[slot];
+#else
+    double* const S = nullptr;
+#endif
     for (int k = lane; k < _numpar_; k += SODECL_G) {
         pp[k] = params[(size_t)ob * _numpar_ + k];
         gp[k] = 0.0;
@@ -95,7 +101,7 @@ sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict
         for (int s = sb; s < se; ++s) {
             double* row = wk + (size_t)(s - sb) * SODECL_WROW;
             if (lane == 0) row[SODECL_NSTAGE * _numeq_] = t;
-            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, row);
+            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, row);
             t = t + _m_dt_;
         }
         __syncthreads();   // workspace rows written by other lanes are read below

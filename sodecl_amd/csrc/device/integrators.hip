@@ -371,6 +371,66 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
 #define SODECL_XSTRIDE (_numeq_ | 1)
 #define SODECL_NSTAGE (SODECL_SOLVER == 2 ? 4 : 1)
 
+// Drift of this lane's components at the published stage vector x.
+//
+// With the plugin's pairwise form (sode_pair, sode_system_pairsum: f_i is
+// finished from s_i = sum_j pair(i, j) taken in j order, pair(i, j) ==
+// -pair(j, i) exactly), every unordered pair is evaluated once: lane of row
+// i evaluates pair(i, i + d mod N) for d = 1 .. (N-1)/2 (and d = N/2 for the
+// first half of the rows when N is even) plus its diagonal, writes the value
+// to S[i][j] and its negation to S[j][i], and after one barrier sums its own
+// row of S in j order -- the same terms in the same order as the per-row
+// loop, with half the evaluations.  Otherwise each component comes from
+// sode_system_i.
+#if SODECL_USE_PAIRS
+#define SODECL_SSTRIDE (_numeq_ + 1)
+SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S, int lane, double (&f)[SODECL_CPL])
+{
+    const int half = (_numeq_ - 1) / 2;
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int i = lane + c * SODECL_G;
+        if (i < _numeq_) {
+            S[i * SODECL_SSTRIDE + i] = sode_pair(i, i, t, x, pp);
+            for (int d = 1; d <= half; ++d) {
+                const int j = (i + d) % _numeq_;
+                const double v = sode_pair(i, j, t, x, pp);
+                S[i * SODECL_SSTRIDE + j] = v;
+                S[j * SODECL_SSTRIDE + i] = -v;
+            }
+            if ((_numeq_ % 2) == 0 && i < _numeq_ / 2) {
+                const int j = i + _numeq_ / 2;
+                const double v = sode_pair(i, j, t, x, pp);
+                S[i * SODECL_SSTRIDE + j] = v;
+                S[j * SODECL_SSTRIDE + i] = -v;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int i = lane + c * SODECL_G;
+        f[c] = 0.0;
+        if (i < _numeq_) {
+            double acc = 0;
+            for (int j = 0; j < _numeq_; ++j) acc = acc + S[i * SODECL_SSTRIDE + j];
+            f[c] = sode_system_pairsum(i, t, x, pp, acc);
+        }
+    }
+}
+#define SODECL_EVAL(T, X, F) sodecl_group_eval(T, X, pp, S, lane, F)
+#else
+SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, int lane, double (&f)[SODECL_CPL])
+{
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int comp = lane + c * SODECL_G;
+        f[c] = comp < _numeq_ ? sode_system_i(comp, t, x, pp) : 0.0;
+    }
+}
+#define SODECL_EVAL(T, X, F) sodecl_group_eval(T, X, pp, lane, F)
+#endif
+
 // Writes this lane's components of v into LDS row x.
 SODECL_INLINE void sodecl_publish(double* x, const double (&v)[SODECL_CPL], int lane)
 {
@@ -395,70 +455,62 @@ SODECL_INLINE void sodecl_store_row(double* row, const double (&v)[SODECL_CPL], 
 #if SODECL_SOLVER == 1
 // Euler (euler.cl:10-18).  stages: null, or one row receiving y.
 SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (*xs)[SODECL_OPB][SODECL_XSTRIDE],
-                                     int& buf, int slot, int lane, double* pp, double* stages)
+                                     int& buf, int slot, int lane, double* pp, double* S, double* stages)
 {
+    double f[SODECL_CPL];
     sodecl_store_row(stages, y, lane);
     sodecl_publish(xs[buf][slot], y, lane);
     __syncthreads();
+    SODECL_EVAL(t, xs[buf][slot], f);
 #pragma unroll
-    for (int c = 0; c < SODECL_CPL; ++c) {
-        const int comp = lane + c * SODECL_G;
-        if (comp < _numeq_) {
-            const double f = sode_system_i(comp, t, xs[buf][slot], pp);
-            y[c] = y[c] + f * _m_dt_;
-        }
-    }
+    for (int c = 0; c < SODECL_CPL; ++c)
+        if (lane + c * SODECL_G < _numeq_) y[c] = y[c] + f[c] * _m_dt_;
     buf ^= 1;
 }
 #elif SODECL_SOLVER == 2
 // Classic RK4 (rk4.cl:17-64), one LDS stage row per evaluation.  stages:
 // null, or four rows receiving x1 = y, x2, x3, x4.
 SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (*xs)[SODECL_OPB][SODECL_XSTRIDE],
-                                     int& buf, int slot, int lane, double* pp, double* stages)
+                                     int& buf, int slot, int lane, double* pp, double* S, double* stages)
 {
-    double acc[SODECL_CPL], xn[SODECL_CPL];
+    double acc[SODECL_CPL], xn[SODECL_CPL], f[SODECL_CPL];
     sodecl_store_row(stages, y, lane);
     sodecl_publish(xs[buf][slot], y, lane);
     __syncthreads();
+    SODECL_EVAL(t, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
-        const int comp = lane + c * SODECL_G;
-        const double f = comp < _numeq_ ? sode_system_i(comp, t, xs[buf][slot], pp) : 0.0;
-        acc[c] = f;
-        xn[c] = y[c] + (_m_dt_ * 0.50) * f;
+        acc[c] = f[c];
+        xn[c] = y[c] + (_m_dt_ * 0.50) * f[c];
     }
     buf ^= 1;
     sodecl_store_row(stages ? stages + _numeq_ : nullptr, xn, lane);
     sodecl_publish(xs[buf][slot], xn, lane);
     __syncthreads();
+    SODECL_EVAL(t + 0.50 * _m_dt_, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
-        const int comp = lane + c * SODECL_G;
-        const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, xs[buf][slot], pp) : 0.0;
-        acc[c] = acc[c] + 2.00 * f;
-        xn[c] = y[c] + (_m_dt_ * 0.50) * f;
+        acc[c] = acc[c] + 2.00 * f[c];
+        xn[c] = y[c] + (_m_dt_ * 0.50) * f[c];
     }
     buf ^= 1;
     sodecl_store_row(stages ? stages + 2 * _numeq_ : nullptr, xn, lane);
     sodecl_publish(xs[buf][slot], xn, lane);
     __syncthreads();
+    SODECL_EVAL(t + 0.50 * _m_dt_, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
-        const int comp = lane + c * SODECL_G;
-        const double f = comp < _numeq_ ? sode_system_i(comp, t + 0.50 * _m_dt_, xs[buf][slot], pp) : 0.0;
-        acc[c] = acc[c] + 2.00 * f;
-        xn[c] = y[c] + _m_dt_ * f;
+        acc[c] = acc[c] + 2.00 * f[c];
+        xn[c] = y[c] + _m_dt_ * f[c];
     }
     buf ^= 1;
     sodecl_store_row(stages ? stages + 3 * _numeq_ : nullptr, xn, lane);
     sodecl_publish(xs[buf][slot], xn, lane);
     __syncthreads();
+    SODECL_EVAL(t + _m_dt_, xs[buf][slot], f);
 #pragma unroll
-    for (int c = 0; c < SODECL_CPL; ++c) {
-        const int comp = lane + c * SODECL_G;
-        const double f = comp < _numeq_ ? sode_system_i(comp, t + _m_dt_, xs[buf][slot], pp) : 0.0;
-        y[c] = y[c] + sodecl_div((acc[c] + f) * _m_dt_, 6.00, 1.0 / 6.00);
-    }
+    for (int c = 0; c < SODECL_CPL; ++c)
+        if (lane + c * SODECL_G < _numeq_) y[c] = y[c] + sodecl_div((acc[c] + f[c]) * _m_dt_, 6.00, 1.0 / 6.00);
     buf ^= 1;
 }
 #endif
@@ -479,11 +531,20 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
 #if SODECL_IS_SDE
     __shared__ double sh_n[2][SODECL_OPB][SODECL_NNOI_ALLOC];
 #endif
+#if SODECL_USE_PAIRS
+    __shared__ double sh_S[SODECL_OPB][_numeq_ * SODECL_SSTRIDE];
+#endif
     const int lane = threadIdx.x % SODECL_G;
     const int slot = threadIdx.x / SODECL_G;
     const int orbit = blockIdx.x * SODECL_OPB + slot;
     const bool active = orbit < orbits;
     (void)newton;
+#if SODECL_USE_PAIRS
+    double* const S = sh_S[slot];
+#else
+    double* const S = nullptr;
+#endif
+    (void)S;
 
     for (int k = lane; k < _numpar_; k += SODECL_G)
         sh_p[slot][k] = active ? params[(size_t)orbit * _numpar_ + k] : 0.0;
@@ -542,18 +603,21 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
             sodecl_publish(sh_x[buf][slot], y, lane);
             __syncthreads();
             // Euler-Maruyama (stochastic_euler.cl:10-22)
+            {
+                double f[SODECL_CPL];
+                SODECL_EVAL(t, sh_x[buf][slot], f);
 #pragma unroll
-            for (int c = 0; c < SODECL_CPL; ++c) {
-                const int comp = lane + c * SODECL_G;
-                if (comp < _numeq_) {
-                    const double f = sode_system_i(comp, t, sh_x[buf][slot], pp);
-                    const double gn = sode_system_stoch_i(comp, t, sh_x[buf][slot], pp, sh_n[buf][slot]);
-                    y[c] = y[c] + f * _m_dt_ + gn;
+                for (int c = 0; c < SODECL_CPL; ++c) {
+                    const int comp = lane + c * SODECL_G;
+                    if (comp < _numeq_) {
+                        const double gn = sode_system_stoch_i(comp, t, sh_x[buf][slot], pp, sh_n[buf][slot]);
+                        y[c] = y[c] + f[c] * _m_dt_ + gn;
+                    }
                 }
             }
             buf ^= 1;
 #else
-            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, nullptr);
+            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, nullptr);
 #endif
             t = t + _m_dt_;
         }

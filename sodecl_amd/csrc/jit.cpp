@@ -76,6 +76,7 @@ PluginFeatures detect_features(const std::string& source)
     f.jacobian_band = defines("calc_jacobian_band");
     f.jacobian_band_i = defines("calc_jacobian_band_i");
     f.vjp_group = defines("sode_system_vjp_group");
+    f.pairs = defines("sode_pair") && defines("sode_system_pairsum");
     return f;
 }
 
@@ -126,6 +127,7 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     o << "#define SODECL_HAS_JACOBIAN_BAND_I " << (f.jacobian_band_i ? 1 : 0) << "\n";
     o << "#define SODECL_HAS_JACOBIAN " << (f.calc_jacobian ? 1 : 0) << "\n";
     o << "#define SODECL_HAS_VJP_GROUP " << (f.vjp_group ? 1 : 0) << "\n";
+    o << "#define SODECL_USE_PAIRS " << (k.schedule == 2 && k.use_pairs && f.pairs ? 1 : 0) << "\n";
     o << k.extra_defines;
     // OpenCL C spellings a system source may use
     for (const char* kw : {"__kernel", "__global", "__private"})

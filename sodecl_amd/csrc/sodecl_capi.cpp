@@ -63,6 +63,7 @@ ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, c
     k.block = block;
     k.newton_max_iter = ctx->opts.newton_max_iter > 0 ? ctx->opts.newton_max_iter : 1000;
     k.fp_contract = ctx->opts.fp_contract;
+    k.use_pairs = (schedule == 2 && ctx->features.pairs && !ctx->opts.disable_pairs) ? 1 : 0;
     k.extra_defines = extra;
     return k;
 }
@@ -192,6 +193,19 @@ int default_group(int neq)
     return std::max(g, 4);
 }
 
+// The pairwise drift keeps an N x (N+1) matrix per orbit in LDS.  Largest
+// block (a power-of-two number of orbits) whose matrices fit in 48 KiB; a
+// large bound when the group schedule does not use the pairwise form.
+int pairs_block_cap(const sodecl_ctx* c, int group)
+{
+    if (!c->features.pairs || c->opts.disable_pairs) return 1 << 20;
+    const long per_orbit = (long)c->neq * (c->neq + 1) * 8;
+    const long opb = std::max(1L, (48L * 1024) / per_orbit);
+    long p2 = 1;
+    while (p2 * 2 <= opb) p2 *= 2;
+    return (int)(p2 * group);
+}
+
 LaunchCfg choose_config(const sodecl_ctx* c)
 {
     LaunchCfg cfg{1, 64, 128};
@@ -210,6 +224,7 @@ LaunchCfg choose_config(const sodecl_ctx* c)
         while (p2 < g) p2 <<= 1;
         cfg.group = p2;
         int b = c->opts.block_size > 0 ? c->opts.block_size : 256;
+        if (c->opts.block_size <= 0) b = std::min(b, pairs_block_cap(c, cfg.group));
         b = std::max(b, cfg.group);
         b = (b / cfg.group) * cfg.group;
         cfg.block = std::min(b, 1024);
@@ -481,9 +496,16 @@ std::vector<LaunchCfg> tune_candidates(const sodecl_ctx* c)
         for (int b : {64, 128, 256}) out.push_back(LaunchCfg{1, 1, b});
     if (group_possible(c)) {
         int g0 = default_group(c->neq);
-        for (int g = g0; g >= std::max(2, g0 / 4); g >>= 1)
+        for (int g = g0; g >= std::max(2, g0 / 4); g >>= 1) {
+            const int cap = pairs_block_cap(c, g);
+            bool any = false;
             for (int b : {64, 256})
-                if (b >= g) out.push_back(LaunchCfg{2, g, b});
+                if (b >= g && b <= cap) {
+                    out.push_back(LaunchCfg{2, g, b});
+                    any = true;
+                }
+            if (!any) out.push_back(LaunchCfg{2, g, std::max(g, std::min(cap, 256))});
+        }
     }
     if (out.empty()) out.push_back(LaunchCfg{1, 1, 128});
     return out;
@@ -906,6 +928,9 @@ size_t sodecl_compile_only(sodecl_ctx* c, const char* arch)
 {
     if (!c) return 0;
     LaunchCfg cfg = choose_config(c);
+    c->used_schedule = cfg.schedule;   // reported by sodecl_get_launch_config
+    c->used_group = cfg.group;
+    c->used_block = cfg.block;
     ProgramKey key = sodecl_amd::make_key(c, cfg.schedule, cfg.group, cfg.block, std::string());
     size_t total = 0;
     std::vector<std::string> variants = {std::string()};

@@ -23,6 +23,7 @@ struct PluginFeatures {
     bool jacobian_band = false;     // calc_jacobian_band(y, lower, diag, upper, p)
     bool jacobian_band_i = false;   // calc_jacobian_band_i(i, y, p, *lower, *diag, *upper)
     bool vjp_group = false;         // sode_system_vjp_group(...) (adjoint, group schedule)
+    bool pairs = false;             // sode_pair + sode_system_pairsum (antisymmetric pairwise drift)
 };
 
 PluginFeatures detect_features(const std::string& source);
@@ -38,6 +39,7 @@ struct ProgramKey {
     int block = 128;
     int newton_max_iter = 1000;
     int fp_contract = 0;
+    int use_pairs = 0;           // group schedule evaluates the drift through the pairwise form
     std::string extra_defines;   // additional "#define" lines (adjoint, batch)
 };
 

@@ -15,6 +15,10 @@
  * Optional entry points read by the MI355X integrators:
  *   sode_system_i        one component of the right-hand side (lets several
  *                        lanes share one orbit, lane i computing component i);
+ *   sode_pair,
+ *   sode_system_pairsum  the same drift in pairwise form: the pair term
+ *                        sin(y_j - y_i) is exactly antisymmetric, so the
+ *                        integrator evaluates each pair once;
  *   sode_system_stoch_i  one component of the diffusion term;
  *   calc_jacobian        dense df/dy (implicit solvers, sensitivity checks);
  *   calc_jacobian_params dense df/dp (sensitivity checks).
@@ -31,6 +35,20 @@ double sode_system_i(int i, double t, double y[_numeq_], double p[_numpar_])
     }
     acc = acc * (double)(p[0] / (double)(KURA_N));
     return acc + p[i + 1];
+}
+
+/* Pairwise form: f_i = (sum over j in order of sode_pair(i, j)) * (K/N) + w_i,
+ * the same operations as sode_system_i.  sin is odd and y_j - y_i is the
+ * exact negation of y_i - y_j, so sode_pair(i, j) == -sode_pair(j, i). */
+double sode_pair(int i, int j, double t, double y[_numeq_], double p[_numpar_])
+{
+    return sin(y[j] - y[i]);
+}
+
+double sode_system_pairsum(int i, double t, double y[_numeq_], double p[_numpar_], double s)
+{
+    s = s * (double)(p[0] / (double)(KURA_N));
+    return s + p[i + 1];
 }
 
 void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])

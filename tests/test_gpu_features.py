@@ -1,7 +1,8 @@
-"""GPU tests of the launch machinery: the autotuner and hipGraph replay.
-Both must leave results unchanged: the tuned configuration is checked
-against the oracle at 1e-10, graph replays against the eager launches bit
-for bit."""
+"""GPU tests of the launch machinery: the autotuner, hipGraph replay and the
+pairwise drift form.  None may change results: the tuned configuration is
+checked against the oracle at 1e-10, graph replays against the eager
+launches and the pairwise form against the per-component form bit for
+bit."""
 import os
 
 import numpy as np
@@ -68,3 +69,47 @@ def test_graph_replay_matches_eager_launches():
         assert torch.equal(y, ye), f"replay {rep}"
     ref = Oracle("oculomotor").run("rk4", cfg["y0"], cfg["params"], cfg["dt"], cfg["tspan"], cfg["ksteps"])["out"]
     assert_traj_close(traj_g.cpu().numpy(), ref, what="graph")
+
+
+@pytest.mark.parametrize("solver,n,group", [(1, 7, 8), (2, 10, 4), (2, 32, 32), (2, 64, 64), (0, 32, 16), (0, 5, 8)])
+def test_pairwise_drift_bit_identical(solver, n, group):
+    """The pairwise form of the Kuramoto drift (each sin(y_j - y_i) evaluated
+    once, its negation mirrored) sums the same terms in the same order as
+    sode_system_i, so trajectories must agree bit for bit."""
+    orbits = 301
+    if solver == 0:
+        cfg = configs.kuramoto_em(orbits=orbits, n=n, seed=n, tspan=0.2, ksteps=10)
+    else:
+        cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=n, dt=0.01, tspan=1.0, ksteps=10)
+        cfg["solver"] = solver
+    outs = []
+    for disable in (0, 1):
+        s = configs.session_for(cfg, schedule="group", group_size=group, disable_pairs=disable)
+        s.setup(cfg["y0"], cfg["params"], None, cfg.get("counters"))
+        outs.append(s.run().copy())
+    assert np.array_equal(outs[0], outs[1])
+    if solver != 0:
+        ref = Oracle("kuramoto").run({1: "euler", 2: "rk4"}[solver], cfg["y0"], cfg["params"], cfg["dt"],
+                                     cfg["tspan"], cfg["ksteps"])["out"]
+        assert_traj_close(outs[0].reshape(-1, n, orbits), ref, what=f"pairs N={n}")
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+def test_pairwise_adjoint_bit_identical(solver):
+    from sodecl_amd.torch_ops import Integrator
+    n, orbits = 24, 45
+    cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=4, dt=0.01, tspan=0.5, ksteps=5)
+    rng = np.random.default_rng(4)
+    res = []
+    for disable in (0, 1):
+        integ = Integrator(cfg["system"], solver, orbits, n, n + 1, cfg["dt"], cfg["tspan"], cfg["ksteps"],
+                           schedule="group", group_size=8, disable_pairs=disable)
+        y0 = torch.tensor(cfg["y0"], device=DEV, requires_grad=True)
+        p = torch.tensor(cfg["params"], device=DEV, requires_grad=True)
+        cot = torch.tensor(rng.normal(size=(integ.nsnap, n, orbits)) if not res else res[0][3], device=DEV)
+        traj = integ(y0, p)
+        (traj * cot).sum().backward()
+        torch.cuda.synchronize()
+        res.append((traj.detach().cpu().numpy(), y0.grad.cpu().numpy(), p.grad.cpu().numpy(), cot.cpu().numpy()))
+    for a, b in zip(res[0][:3], res[1][:3]):
+        assert np.array_equal(a, b)
