@@ -77,8 +77,9 @@ typedef struct sodecl_options {
     int fp_contract;         /* 0 = fused multiply-add contraction allowed (default), 1 = off */
     int verbose;             /* print "Compute results runtime" like the reference */
     size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
-    int disable_pairs;       /* 1: under the group schedule evaluate every component with sode_system_i even
-                                when the plugin has the pairwise form (sode_pair, sode_system_pairsum) */
+    int pairwise;            /* group schedule with a plugin that defines the pairwise drift (sode_pair,
+                                sode_system_pairsum): 0 = automatic (used while the per-orbit N x (N+1)
+                                LDS matrix is at most 16 KiB), 1 = always, -1 = never */
 } sodecl_options;
 
 typedef struct sodecl_ctx sodecl_ctx;

@@ -29,7 +29,7 @@ SODECL_INLINE void sodecl_group_vjp(double t, const double (&v)[SODECL_CPL], con
 {
     sodecl_publish(sv[vb][slot], v, lane);
     sodecl_publish(sl[vb][slot], l, lane);
-    __syncthreads();
+    sodecl_group_sync();
     sode_system_vjp_group(t, sv[vb][slot], sl[vb][slot], pp, sg, gp);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {

@@ -358,15 +358,28 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
 // ---------------------------------------------------------------------------
 // Group schedule: SODECL_GROUP lanes per orbit (explicit solvers).
 // Lane l of a group owns components l, l + G, l + 2G, ...; each stage vector
-// is written to an LDS row, one barrier publishes it, and every lane then
+// is written to an LDS row, one wavefront barrier publishes it, and every lane then
 // evaluates its components with the plugin's per-component entry point,
 // reading the whole stage vector from LDS (same-address reads broadcast).
-// Rows are double-buffered so one barrier per stage suffices.
+// Rows are double-buffered so one barrier per stage suffices.  (The group
+// never spans wavefronts, so no barrier waits on another wavefront.)
 // ---------------------------------------------------------------------------
 #if SODECL_SCHEDULE == 2 && SODECL_HAS_COMPONENT && !SODECL_IS_IMPLICIT && (!SODECL_IS_SDE || SODECL_HAS_STOCH_COMPONENT)
 
 #define SODECL_G SODECL_GROUP
 #define SODECL_CPL ((_numeq_ + SODECL_G - 1) / SODECL_G)
+
+// G is a power of two no larger than the wavefront, so the lanes of a group
+// always sit in one wavefront.  Their exchanges through LDS need only
+// wavefront-scope ordering: DS instructions of one wavefront execute in
+// order, and the fences plus the wave barrier keep the compiler from moving
+// LDS accesses across this point.  Waves of a block run independently.
+SODECL_INLINE void sodecl_group_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 #define SODECL_OPB (SODECL_BLOCK / SODECL_G)
 #define SODECL_XSTRIDE (_numeq_ | 1)
 #define SODECL_NSTAGE (SODECL_SOLVER == 2 ? 4 : 1)
@@ -406,7 +419,7 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
             }
         }
     }
-    __syncthreads();
+    sodecl_group_sync();
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
         const int i = lane + c * SODECL_G;
@@ -460,7 +473,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
     double f[SODECL_CPL];
     sodecl_store_row(stages, y, lane);
     sodecl_publish(xs[buf][slot], y, lane);
-    __syncthreads();
+    sodecl_group_sync();
     SODECL_EVAL(t, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c)
@@ -476,7 +489,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
     double acc[SODECL_CPL], xn[SODECL_CPL], f[SODECL_CPL];
     sodecl_store_row(stages, y, lane);
     sodecl_publish(xs[buf][slot], y, lane);
-    __syncthreads();
+    sodecl_group_sync();
     SODECL_EVAL(t, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
@@ -486,7 +499,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
     buf ^= 1;
     sodecl_store_row(stages ? stages + _numeq_ : nullptr, xn, lane);
     sodecl_publish(xs[buf][slot], xn, lane);
-    __syncthreads();
+    sodecl_group_sync();
     SODECL_EVAL(t + 0.50 * _m_dt_, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
@@ -496,7 +509,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
     buf ^= 1;
     sodecl_store_row(stages ? stages + 2 * _numeq_ : nullptr, xn, lane);
     sodecl_publish(xs[buf][slot], xn, lane);
-    __syncthreads();
+    sodecl_group_sync();
     SODECL_EVAL(t + 0.50 * _m_dt_, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
@@ -506,7 +519,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
     buf ^= 1;
     sodecl_store_row(stages ? stages + 3 * _numeq_ : nullptr, xn, lane);
     sodecl_publish(xs[buf][slot], xn, lane);
-    __syncthreads();
+    sodecl_group_sync();
     SODECL_EVAL(t + _m_dt_, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c)
@@ -601,7 +614,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
             }
             cnt = cnt + (double)calls;
             sodecl_publish(sh_x[buf][slot], y, lane);
-            __syncthreads();
+            sodecl_group_sync();
             // Euler-Maruyama (stochastic_euler.cl:10-22)
             {
                 double f[SODECL_CPL];

@@ -43,7 +43,7 @@ void apply_options(sodecl_options& o, const py::dict& d)
         else if (k == "fp_contract") o.fp_contract = kv.second.cast<int>();
         else if (k == "verbose") o.verbose = kv.second.cast<int>();
         else if (k == "traj_budget_bytes") o.traj_budget_bytes = kv.second.cast<size_t>();
-        else if (k == "disable_pairs") o.disable_pairs = kv.second.cast<int>();
+        else if (k == "pairwise") o.pairwise = kv.second.cast<int>();
         else throw std::invalid_argument("unknown option " + k);
     }
 }

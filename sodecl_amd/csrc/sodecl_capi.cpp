@@ -48,6 +48,17 @@ bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what)
     return false;
 }
 
+// Whether the group schedule evaluates the drift in the plugin's pairwise
+// form.  Automatic choice: only while the per-orbit LDS matrix stays within
+// 16 KiB (N <= 44); larger matrices limit the kernel to about one wavefront
+// per SIMD, which costs more than the halved evaluations save.
+bool pairs_active(const sodecl_ctx* c)
+{
+    if (!c->features.pairs || c->opts.pairwise < 0) return false;
+    if (c->opts.pairwise > 0) return true;
+    return (long)c->neq * (c->neq + 1) * 8 <= 16 * 1024;
+}
+
 ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra)
 {
     ProgramKey k;
@@ -63,7 +74,7 @@ ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, c
     k.block = block;
     k.newton_max_iter = ctx->opts.newton_max_iter > 0 ? ctx->opts.newton_max_iter : 1000;
     k.fp_contract = ctx->opts.fp_contract;
-    k.use_pairs = (schedule == 2 && ctx->features.pairs && !ctx->opts.disable_pairs) ? 1 : 0;
+    k.use_pairs = (schedule == 2 && pairs_active(ctx)) ? 1 : 0;
     k.extra_defines = extra;
     return k;
 }
@@ -198,7 +209,7 @@ int default_group(int neq)
 // large bound when the group schedule does not use the pairwise form.
 int pairs_block_cap(const sodecl_ctx* c, int group)
 {
-    if (!c->features.pairs || c->opts.disable_pairs) return 1 << 20;
+    if (!pairs_active(c)) return 1 << 20;
     const long per_orbit = (long)c->neq * (c->neq + 1) * 8;
     const long opb = std::max(1L, (48L * 1024) / per_orbit);
     long p2 = 1;

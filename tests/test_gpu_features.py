@@ -83,8 +83,8 @@ def test_pairwise_drift_bit_identical(solver, n, group):
         cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=n, dt=0.01, tspan=1.0, ksteps=10)
         cfg["solver"] = solver
     outs = []
-    for disable in (0, 1):
-        s = configs.session_for(cfg, schedule="group", group_size=group, disable_pairs=disable)
+    for mode in (1, -1):
+        s = configs.session_for(cfg, schedule="group", group_size=group, pairwise=mode)
         s.setup(cfg["y0"], cfg["params"], None, cfg.get("counters"))
         outs.append(s.run().copy())
     assert np.array_equal(outs[0], outs[1])
@@ -101,9 +101,9 @@ def test_pairwise_adjoint_bit_identical(solver):
     cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=4, dt=0.01, tspan=0.5, ksteps=5)
     rng = np.random.default_rng(4)
     res = []
-    for disable in (0, 1):
+    for mode in (1, -1):
         integ = Integrator(cfg["system"], solver, orbits, n, n + 1, cfg["dt"], cfg["tspan"], cfg["ksteps"],
-                           schedule="group", group_size=8, disable_pairs=disable)
+                           schedule="group", group_size=8, pairwise=mode)
         y0 = torch.tensor(cfg["y0"], device=DEV, requires_grad=True)
         p = torch.tensor(cfg["params"], device=DEV, requires_grad=True)
         cot = torch.tensor(rng.normal(size=(integ.nsnap, n, orbits)) if not res else res[0][3], device=DEV)

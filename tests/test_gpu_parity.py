@@ -147,10 +147,15 @@ def test_second_run_continues():
     a = s.run().reshape(s.nsnap, 6, 64)
     b = s.run().reshape(s.nsnap, 6, 64)
     np.testing.assert_array_equal(a[-1], b[0])
+    # The oracle continued the same way: a second run of ceil(tspan / (dt k))
+    # launches from its own final state.  (Two runs are not one run over
+    # 2 * tspan: here ceil(0.01 / 2e-4) = 51 in FP64, twice that is 102, and
+    # ceil(0.02 / 2e-4) = 101.)
+    ref1 = oracle_run(cfg, "oculomotor")["out"]
     cfg2 = dict(cfg)
-    cfg2["tspan"] = 2 * cfg["tspan"]
-    ref = oracle_run(cfg2, "oculomotor")["out"]
-    assert_traj_close(b[-1][None], ref[-1][None], what="continued run")
+    cfg2["y0"] = np.ascontiguousarray(ref1[-1].T)
+    ref2 = oracle_run(cfg2, "oculomotor")["out"]
+    assert_traj_close(b[-1][None], ref2[-1][None], what="continued run")
 
 
 def test_chunked_trajectory_matches_single_launch():
