@@ -222,13 +222,25 @@ SODECL_INLINE int sodecl_implicit_step(double t, double* y, double* yout, double
 #endif  // SODECL_IS_IMPLICIT
 
 // ---------------------------------------------------------------------------
-// One step of the selected solver, lane schedule (whole state per lane).
+// One step of the selected solver with the whole state per lane (lane and
+// team schedules).  Under the team schedule the drift comes from the
+// plugin's team entry point, which the lanes of a team evaluate together.
 // ---------------------------------------------------------------------------
+// Optional attributes of the integration kernels (e.g. occupancy hints)
+#ifndef SODECL_KERNEL_ATTR
+#define SODECL_KERNEL_ATTR
+#endif
+
+#if SODECL_SCHEDULE == 4
+#define SODECL_RHS sode_system_team
+#else
+#define SODECL_RHS sode_system
+#endif
 #if SODECL_SOLVER == 1
 // Euler (euler.cl:10-18)
 SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
 {
-    sode_system(t, y, yout, p);
+    SODECL_RHS(t, y, yout, p);
     SODECL_UNROLL
     for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + yout[i] * _m_dt_;
     return 0;
@@ -239,22 +251,22 @@ SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
 SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
 {
     double k[_numeq_], x[_numeq_];
-    sode_system(t, y, yout, p);
+    SODECL_RHS(t, y, yout, p);
     SODECL_UNROLL
     for (int i = 0; i < _numeq_; ++i) x[i] = y[i] + (_m_dt_ * 0.50) * yout[i];
-    sode_system(t + 0.50 * _m_dt_, x, k, p);
+    SODECL_RHS(t + 0.50 * _m_dt_, x, k, p);
     SODECL_UNROLL
     for (int i = 0; i < _numeq_; ++i) {
         yout[i] = yout[i] + 2.00 * k[i];
         x[i] = y[i] + (_m_dt_ * 0.50) * k[i];
     }
-    sode_system(t + 0.50 * _m_dt_, x, k, p);
+    SODECL_RHS(t + 0.50 * _m_dt_, x, k, p);
     SODECL_UNROLL
     for (int i = 0; i < _numeq_; ++i) {
         yout[i] = yout[i] + 2.00 * k[i];
         x[i] = y[i] + _m_dt_ * k[i];
     }
-    sode_system(t + _m_dt_, x, k, p);
+    SODECL_RHS(t + _m_dt_, x, k, p);
     SODECL_UNROLL
     for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + sodecl_div((yout[i] + k[i]) * _m_dt_, 6.00, 1.0 / 6.00);
     return 0;
@@ -275,7 +287,7 @@ SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
 //   newton: per-orbit Newton update counts (implicit solvers) or null.
 // ---------------------------------------------------------------------------
 #if SODECL_SCHEDULE == 1
-extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
+extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
 sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
             double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
             int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
@@ -354,6 +366,70 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
     if (newton) newton[i] += nu;
 }
 #endif  // SODECL_SCHEDULE == 1
+
+// ---------------------------------------------------------------------------
+// Team schedule kernel: SODECL_GROUP lanes per orbit, each holding the whole
+// state and stepping it with the same arithmetic as the lane schedule; the
+// plugin's sode_system_team shares the drift's costly terms among the team.
+// This raises the number of wavefronts for small orbit counts (65536 orbits
+// fill only one wavefront per SIMD in the lane schedule) at the cost of the
+// terms the team duplicates.  Lane l of a team stores components m with
+// m % G == l.  Arguments as sodecl_lane (explicit, non-stochastic solvers).
+// ---------------------------------------------------------------------------
+#if SODECL_SCHEDULE == 4 && (SODECL_SOLVER == 1 || SODECL_SOLVER == 2)
+extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
+sodecl_team(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
+            double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
+            int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
+            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0)
+{
+    const int gid = blockIdx.x * SODECL_BLOCK + threadIdx.x;
+    const int i = gid / SODECL_GROUP;            // orbit: a team exits together
+    const int lane = gid % SODECL_GROUP;
+    if (i >= orbits) return;
+    (void)counters;
+    (void)newton;
+    double y[_numeq_], yn[_numeq_], p[SODECL_NPAR_ALLOC];
+    SODECL_UNROLL
+    for (int m = 0; m < _numeq_; ++m) y[m] = y_state[(size_t)i * _numeq_ + m];
+    for (int m = 0; m < _numpar_; ++m) p[m] = params[(size_t)i * _numpar_ + m];
+    double t = t_state[i];
+    const size_t slab = (size_t)_numeq_ * (size_t)orbits;
+    long long ck = ckpt_y ? (gstep0 + ckpt_every - 1) / ckpt_every : 0;
+    long long ck_step = ck * ckpt_every;
+    long long g = gstep0;
+    for (int s = 0; s < nseg; ++s) {
+        if (traj) {
+            double* o = traj + (size_t)(s0 + s) * slab + i;
+            SODECL_UNROLL
+            for (int m = 0; m < _numeq_; ++m)
+                if (m % SODECL_GROUP == lane) o[(size_t)m * orbits] = y[m];
+        }
+        for (int it = 0; it < _numsteps_; ++it, ++g) {
+            if (ckpt_y && g == ck_step) {
+                const long long k = ck++;
+                ck_step += ckpt_every;
+                SODECL_UNROLL
+                for (int m = 0; m < _numeq_; ++m)
+                    if (m % SODECL_GROUP == lane) ckpt_y[(size_t)k * slab + (size_t)m * orbits + i] = y[m];
+                if (lane == 0) ckpt_t[(size_t)k * orbits + i] = t;
+            }
+            sodecl_lane_step(t, y, yn, p);
+            t = t + _m_dt_;
+            SODECL_UNROLL
+            for (int m = 0; m < _numeq_; ++m) y[m] = yn[m];
+        }
+    }
+    SODECL_UNROLL
+    for (int m = 0; m < _numeq_; ++m) {
+        if (m % SODECL_GROUP == lane) {
+            if (traj && final_snap) traj[(size_t)(s0 + nseg) * slab + (size_t)m * orbits + i] = y[m];
+            y_state[(size_t)i * _numeq_ + m] = y[m];
+        }
+    }
+    if (lane == 0) t_state[i] = t;
+}
+#endif  // SODECL_SCHEDULE == 4
 
 // ---------------------------------------------------------------------------
 // Group schedule: SODECL_GROUP lanes per orbit (explicit solvers).
@@ -533,7 +609,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
 // step g with g % ckpt_every == 0 is written to ckpt_y[g/ckpt_every][neq][orbits]
 // and ckpt_t[g/ckpt_every][orbits]; gstep0 is the global index of this
 // launch's first step.
-extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
+extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
 sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
              double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
              int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,

@@ -20,9 +20,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <mutex>
 #include <regex>
-#include <sstream>
 
 #include "sodecl_internal.hpp"
 
@@ -77,6 +77,7 @@ PluginFeatures detect_features(const std::string& source)
     f.jacobian_band_i = defines("calc_jacobian_band_i");
     f.vjp_group = defines("sode_system_vjp_group");
     f.pairs = defines("sode_pair") && defines("sode_system_pairsum");
+    f.team = defines("sode_system_team");
     return f;
 }
 
@@ -271,6 +272,11 @@ bool compile_program(const std::string& text, const std::string& arch, int fp_co
                                      fp_contract ? "-ffp-contract=off" : "-ffp-contract=fast",
                                      // no packed FP32 instructions in any generated code
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    // extra compiler options, whitespace-separated (part of the cache key)
+    if (const char* extra = std::getenv("SODECL_AMD_JIT_OPTIONS")) {
+        std::istringstream in(extra);
+        for (std::string w; in >> w;) opts.push_back(w);
+    }
     std::string okey;
     for (auto& s : opts) okey += s + " ";
     okey += "hiprtc" + std::to_string(vmaj) + "." + std::to_string(vmin);

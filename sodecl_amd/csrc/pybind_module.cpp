@@ -30,6 +30,7 @@ void apply_options(sodecl_options& o, const py::dict& d)
                 o.schedule = v == "lane" ? SODECL_SCHEDULE_LANE
                              : v == "group" ? SODECL_SCHEDULE_GROUP
                              : v == "plain" ? SODECL_SCHEDULE_PLAIN
+                             : v == "team"  ? SODECL_SCHEDULE_TEAM
                                             : SODECL_SCHEDULE_AUTO;
             } else {
                 o.schedule = kv.second.cast<int>();

@@ -183,10 +183,26 @@ void free_device(sodecl_ctx* c)
 
 // Launch configuration for this context.
 struct LaunchCfg {
-    int schedule;   // 1 lane, 2 group
+    int schedule;   // 1 lane, 2 group, 4 team
     int group;
     int block;
 };
+
+const char* schedule_name(int s) { return s == 2 ? "group" : (s == 4 ? "team" : "lane"); }
+
+bool team_possible(const sodecl_ctx* c)
+{
+    return c->features.team && (c->solver == SODECL_EULER || c->solver == SODECL_RUNGE_KUTTA);
+}
+
+// Compute units of the context's device (256 on MI355X when it cannot be read).
+int device_cus(const sodecl_ctx* c)
+{
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus;
+}
 
 bool group_possible(const sodecl_ctx* c)
 {
@@ -224,11 +240,26 @@ LaunchCfg choose_config(const sodecl_ctx* c)
     if (c->opts.schedule == SODECL_SCHEDULE_AUTO && c->tuned_schedule)
         return LaunchCfg{c->tuned_schedule, c->tuned_group, c->tuned_block};
     int sched = c->opts.schedule;
-    if (sched == SODECL_SCHEDULE_AUTO) sched = (group_possible(c) && c->neq >= 8) ? 2 : 1;
+    if (sched == SODECL_SCHEDULE_AUTO) {
+        sched = (group_possible(c) && c->neq >= 8) ? 2 : 1;
+        // fewer orbits than two wavefronts per SIMD in the lane schedule
+        if (sched == 1 && team_possible(c) && (long)c->orbits < 2L * 64 * 4 * device_cus(c)) sched = 4;
+    }
     if (sched == SODECL_SCHEDULE_GROUP && !group_possible(c)) sched = 1;
+    if (sched == SODECL_SCHEDULE_TEAM && !team_possible(c)) sched = 1;
     if (sched == SODECL_SCHEDULE_PLAIN) sched = 1;
     cfg.schedule = sched;
-    if (sched == 2) {
+    if (sched == 4) {
+        int g = c->opts.group_size > 0 ? c->opts.group_size : 2;
+        g = std::min(std::max(g, 2), 64);
+        int p2 = 1;
+        while (p2 < g) p2 <<= 1;
+        cfg.group = p2;
+        int b = c->opts.block_size > 0 ? c->opts.block_size : 128;
+        b = std::max(b, cfg.group);
+        b = (b / cfg.group) * cfg.group;
+        cfg.block = std::min(b, 1024);
+    } else if (sched == 2) {
         int g = c->opts.group_size > 0 ? c->opts.group_size : default_group(c->neq);
         g = std::min(std::max(g, 1), 64);
         int p2 = 1;
@@ -255,12 +286,12 @@ int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const dou
 {
     auto prog = get_program(c, cfg.schedule, cfg.group, cfg.block);
     if (!prog) return 0;
-    const char* kname = cfg.schedule == 2 ? "sodecl_group" : "sodecl_lane";
+    const char* kname = cfg.schedule == 2 ? "sodecl_group" : (cfg.schedule == 4 ? "sodecl_team" : "sodecl_lane");
     hipFunction_t fn = prog->get(kname);
     if (!fn) return fail(c, std::string("kernel ") + kname + " missing from the program");
     int orbits = c->orbits;
     unsigned grid;
-    if (cfg.schedule == 2) {
+    if (cfg.schedule == 2 || cfg.schedule == 4) {
         const int opb = cfg.block / cfg.group;
         grid = (unsigned)((orbits + opb - 1) / opb);
     } else {
@@ -518,6 +549,8 @@ std::vector<LaunchCfg> tune_candidates(const sodecl_ctx* c)
             if (!any) out.push_back(LaunchCfg{2, g, std::max(g, std::min(cap, 256))});
         }
     }
+    if (team_possible(c))
+        for (int b : {64, 128, 256}) out.push_back(LaunchCfg{4, 2, b});
     if (out.empty()) out.push_back(LaunchCfg{1, 1, 128});
     return out;
 }
@@ -558,7 +591,7 @@ int autotune(sodecl_ctx* c)
     {
         std::ifstream f(path);
         int s = 0, g = 0, b = 0;
-        if (f >> s >> g >> b && (s == 1 || s == 2) && b > 0) {
+        if (f >> s >> g >> b && (s == 1 || s == 2 || s == 4) && b > 0) {
             c->tuned_schedule = s;
             c->tuned_group = g;
             c->tuned_block = b;
@@ -592,7 +625,7 @@ int autotune(sodecl_ctx* c)
             hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
             ms_best = std::min(ms_best, ms);
         }
-        rep << (cfg.schedule == 2 ? "group" : "lane") << " G=" << cfg.group << " B=" << cfg.block << ": "
+        rep << schedule_name(cfg.schedule) << " G=" << cfg.group << " B=" << cfg.block << ": "
             << ms_best << " ms\n";
         if (ms_best < best) {
             best = ms_best;
@@ -801,7 +834,7 @@ int sodecl_get_rng_counters(sodecl_ctx* c, double* counters)
 int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
 {
     if (!c || !o) return 0;
-    if (o->schedule < 0 || o->schedule > 3) return fail(c, "invalid schedule");
+    if (o->schedule < 0 || o->schedule > SODECL_SCHEDULE_TEAM) return fail(c, "invalid schedule");
     if (o->group_size < 0 || o->group_size > 64) return fail(c, "group_size must be in 0..64");
     if (o->block_size < 0 || o->block_size > 1024) return fail(c, "block_size must be in 0..1024");
     bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract;

@@ -24,6 +24,7 @@ struct PluginFeatures {
     bool jacobian_band_i = false;   // calc_jacobian_band_i(i, y, p, *lower, *diag, *upper)
     bool vjp_group = false;         // sode_system_vjp_group(...) (adjoint, group schedule)
     bool pairs = false;             // sode_pair + sode_system_pairsum (antisymmetric pairwise drift)
+    bool team = false;              // sode_system_team(...) (team schedule)
 };
 
 PluginFeatures detect_features(const std::string& source);

@@ -20,9 +20,11 @@
  *
  * Plugin contract: OpenCL-C style functions over fixed-size arrays whose
  * extents are the macros _numeq_, _numpar_ and _numnoi_ supplied by the
- * integrator.  calc_jacobian_band is an optional entry point read by the
- * implicit solvers: it returns only the three diagonals the tridiagonal
- * solve uses (lower[i] = J[i+1][i], diag[i] = J[i][i], upper[i] = J[i][i+1]).
+ * integrator.  Optional entry points: calc_jacobian_band, read by the
+ * implicit solvers, returns only the three diagonals the tridiagonal solve
+ * uses (lower[i] = J[i+1][i], diag[i] = J[i][i], upper[i] = J[i][i+1]);
+ * sode_system_team, read by the team schedule, shares the two burst curves
+ * between the two lanes of a pair.
  */
 
 #define OCU_T1 0.15
@@ -67,7 +69,9 @@ double ocu_burst_dx(double x, double a, double b, double at, double bt)
     return pos ? (at * e) / bt : -((a * e) / b + (a * x * e) / (b * b));
 }
 
-void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
+/* The right-hand side given the two burst-curve values br = B(y5) and
+ * bl = B(-y5). */
+void ocu_rhs(double y[_numeq_], double yout[_numeq_], double p[_numpar_], double br, double bl)
 {
     const double rate_sum = (1.00 / OCU_T1) + (1.00 / OCU_T2);
     const double rate_prod = 1.00 / (OCU_T1 * OCU_T2);
@@ -76,10 +80,30 @@ void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_nu
     yout[0] = y[1];
     yout[1] = -(rate_sum * y[1]) - (rate_prod * y[0]) + (rate_prod * y[2]) + rate_sum * drive;
     yout[2] = -ocu_div(y[2], OCU_TN, 1.0 / OCU_TN) + drive;
-    yout[3] = (1.00 / p[2]) * (-y[3] - (p[3] * y[3] * y[4] * y[4]) + ocu_burst(y[5], p[0], p[1], p[4], p[5]));
-    yout[4] = (1.00 / p[2]) * (-y[4] - (p[3] * y[4] * y[3] * y[3]) + ocu_burst(-y[5], p[0], p[1], p[4], p[5]));
+    yout[3] = (1.00 / p[2]) * (-y[3] - (p[3] * y[3] * y[4] * y[4]) + br);
+    yout[4] = (1.00 / p[2]) * (-y[4] - (p[3] * y[4] * y[3] * y[3]) + bl);
     yout[5] = -drive;
 }
+
+void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
+{
+    ocu_rhs(y, yout, p, ocu_burst(y[5], p[0], p[1], p[4], p[5]), ocu_burst(-y[5], p[0], p[1], p[4], p[5]));
+}
+
+#ifdef SODECL_TEAM_API
+/* Team form of sode_system for the team schedule: the two burst curves, each
+ * an exponential and most of the cost of the right-hand side, go to the two
+ * lanes of a pair (even lane B(y5), odd lane B(-y5)) and are swapped between
+ * them; every lane then forms the whole right-hand side.  The operations are
+ * those of sode_system. */
+void sode_system_team(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
+{
+    const int odd = sodecl_lane() & 1;
+    const double mine = ocu_burst(odd ? -y[5] : y[5], p[0], p[1], p[4], p[5]);
+    const double other = sodecl_group_xor(mine, 1);
+    ocu_rhs(y, yout, p, odd ? other : mine, odd ? mine : other);
+}
+#endif
 
 void sode_system_stoch(double t, double y[_numeq_], double stoch[_numeq_], double p[_numpar_], double noise[_numnoi_])
 {

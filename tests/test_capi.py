@@ -193,3 +193,30 @@ def test_pairwise_form_selected_and_lds_bounded():
     lane = si.Session(path, 2, 1000, 10, 11, 0, 1e-3, 1.0, 10, 0, {"schedule": "lane"})
     lane.compile_only("gfx950")
     assert "#define SODECL_USE_PAIRS 0" in lane.program_source()
+
+
+def test_team_schedule_selection():
+    """The team schedule needs the plugin's sode_system_team (oculomotor has
+    it, Kuramoto does not) and an explicit non-stochastic solver; automatic
+    selection takes it while the orbits fill fewer than two wavefronts per
+    SIMD in the lane schedule (256 compute units assumed without a GPU)."""
+    ocu = sodecl_amd.system_path("oculomotor")
+    for solver in (1, 2):
+        s = si.Session(ocu, solver, 65536, 6, 6, 0, 1e-3, 1.0, 10, 0, {})
+        assert s.compile_only("gfx950") > 1000
+        assert s.launch_config() == (4, 2, 128)
+        assert "\nsodecl_team(" in s.program_source()
+    big = si.Session(ocu, 2, 300000, 6, 6, 0, 1e-3, 1.0, 10, 0, {})
+    big.compile_only("gfx950")
+    assert big.launch_config()[0] == 1
+    for solver, nnoi, npar in ((3, 0, 6), (4, 0, 6), (0, 2, 8)):
+        s = si.Session(ocu, solver, 300, 6, npar, nnoi, 1e-3, 1.0, 10, 0, {"schedule": "team"})
+        s.compile_only("gfx950")
+        assert s.launch_config()[0] == 1
+    k = si.Session(sodecl_amd.system_path("kuramoto"), 2, 300, 6, 7, 0, 1e-3, 1.0, 10, 0, {"schedule": "team"})
+    k.compile_only("gfx950")
+    assert k.launch_config()[0] == 1
+    t4 = si.Session(ocu, 2, 300, 6, 6, 0, 1e-3, 1.0, 10, 0, {"schedule": "team", "group_size": 4,
+                                                            "block_size": 256})
+    t4.compile_only("gfx950")
+    assert t4.launch_config() == (4, 4, 256)

@@ -113,3 +113,57 @@ def test_pairwise_adjoint_bit_identical(solver):
         res.append((traj.detach().cpu().numpy(), y0.grad.cpu().numpy(), p.grad.cpu().numpy(), cot.cpu().numpy()))
     for a, b in zip(res[0][:3], res[1][:3]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("orbits,group,block", [(301, 2, 128), (64, 2, 64), (1000, 4, 256)])
+def test_team_schedule_bit_identical_to_lane(solver, orbits, group, block):
+    """The team schedule steps every orbit with the lane schedule's
+    arithmetic; the oculomotor team entry point only moves one of the two
+    burst-curve evaluations to the partner lane.  Trajectories, final states
+    and continued runs must agree bit for bit, and with the oracle at 1e-10
+    (well-conditioned step size)."""
+    cfg = configs.oculomotor_rk4(orbits=orbits, seed=orbits + solver, dt=2e-6, tspan=0.004, ksteps=100)
+    cfg["solver"] = solver
+    cfg["y0"][:, 5] += 1.0
+    outs = []
+    for opts in (dict(schedule="lane"), dict(schedule="team", group_size=group, block_size=block)):
+        s = configs.session_for(cfg, **opts)
+        s.setup(cfg["y0"], cfg["params"])
+        first = s.run().copy()
+        second = s.run().copy()
+        outs.append((first, second, s.launch_config()))
+    assert outs[1][2] == (4, group, block)
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    ref = Oracle("oculomotor").run({1: "euler", 2: "rk4"}[solver], cfg["y0"], cfg["params"], cfg["dt"],
+                                   cfg["tspan"], cfg["ksteps"])["out"]
+    assert_traj_close(outs[1][0].reshape(-1, 6, orbits), ref, what=f"team G={group}")
+
+
+def test_team_schedule_device_run_with_checkpoints_matches_host_run():
+    cfg = configs.oculomotor_rk4(orbits=517, seed=2, dt=2e-6, tspan=0.004, ksteps=100)
+    cfg["y0"][:, 5] += 1.0
+    s = configs.session_for(cfg, schedule="team")
+    s.setup(cfg["y0"], cfg["params"])
+    host = s.run().reshape(s.nsnap, 6, 517)
+    y = torch.tensor(cfg["y0"], device=DEV)
+    p = torch.tensor(cfg["params"], device=DEV)
+    t = torch.zeros(517, dtype=torch.float64, device=DEV)
+    traj = torch.empty((s.nsnap, 6, 517), dtype=torch.float64, device=DEV)
+    every = 7
+    k = s.checkpoint_count(every)
+    ck_t = torch.empty((k, 517), dtype=torch.float64, device=DEV)
+    ck_y = torch.empty((k, 6, 517), dtype=torch.float64, device=DEV)
+    s.run_device_checkpointed(t.data_ptr(), y.data_ptr(), p.data_ptr(), 0, traj.data_ptr(), ck_t.data_ptr(),
+                              ck_y.data_ptr(), every, 0)
+    torch.cuda.synchronize()
+    assert s.launch_config()[0] == 4
+    tr = traj.cpu().numpy()
+    assert np.array_equal(tr, host)
+    # checkpoint j is the state before step j * every; snapshots are every 100 steps
+    cy = ck_y.cpu().numpy()
+    for j in range(k):
+        if (j * every) % 100 == 0:
+            assert np.array_equal(cy[j], tr[(j * every) // 100])
+    assert np.allclose(ck_t.cpu().numpy()[3], 3 * every * cfg["dt"], rtol=1e-12)
