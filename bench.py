@@ -211,8 +211,9 @@ def batch_bench(solver, reps=1):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(orbits shard with no per-step communication, see DESIGN.md)"}))
+        print(json.dumps({"error": "this library is measured on one GPU; --gpus above 1 is not supported "
+                                   "(orbits are independent: each GPU would integrate its own orbit range "
+                                   "with no per-step communication)"}))
         sys.exit(2)
     import numpy as np
     import torch

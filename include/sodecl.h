@@ -53,10 +53,8 @@ enum sodecl_output { SODECL_OUTPUT_NONE = 0, SODECL_OUTPUT_ARRAY = 1, SODECL_OUT
  * work-item per orbit and one launch per ksteps segment). */
 enum sodecl_schedule {
     SODECL_SCHEDULE_AUTO = 0,  /* group when the plugin has per-component entry points
-                                  and the system is large enough; team when the plugin
-                                  has sode_system_team and the orbits fill fewer than two
-                                  wavefronts per SIMD; else lane; timed choice when
-                                  autotune is on */
+                                  and the system is large enough, else lane; timed
+                                  choice (team included) when autotune is on */
     SODECL_SCHEDULE_LANE = 1,  /* one orbit per lane, state in registers */
     SODECL_SCHEDULE_GROUP = 2, /* group_size lanes per orbit, stage vectors in LDS */
     SODECL_SCHEDULE_PLAIN = 3, /* the reference's launch design: one launch per ksteps

@@ -371,9 +371,11 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
 // Team schedule kernel: SODECL_GROUP lanes per orbit, each holding the whole
 // state and stepping it with the same arithmetic as the lane schedule; the
 // plugin's sode_system_team shares the drift's costly terms among the team.
-// This raises the number of wavefronts for small orbit counts (65536 orbits
-// fill only one wavefront per SIMD in the lane schedule) at the cost of the
-// terms the team duplicates.  Lane l of a team stores components m with
+// This multiplies the wavefronts of a launch by G at the cost of the work the
+// team duplicates, which pays only when the shared terms dominate the step
+// and the orbits leave SIMDs idle (for the oculomotor model, whose two burst
+// curves are about half of an RK4 step, it does not: 4.4e10 against 6.1e10
+// orbit-steps/s for 65536 orbits).  Lane l of a team stores components m with
 // m % G == l.  Arguments as sodecl_lane (explicit, non-stochastic solvers).
 // ---------------------------------------------------------------------------
 #if SODECL_SCHEDULE == 4 && (SODECL_SOLVER == 1 || SODECL_SOLVER == 2)

@@ -59,6 +59,15 @@ bool pairs_active(const sodecl_ctx* c)
     return (long)c->neq * (c->neq + 1) * 8 <= 16 * 1024;
 }
 
+// Compute units of the context's device (256 on MI355X when it cannot be read).
+int device_cus(const sodecl_ctx* c)
+{
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus;
+}
+
 ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra)
 {
     ProgramKey k;
@@ -75,6 +84,13 @@ ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, c
     k.newton_max_iter = ctx->opts.newton_max_iter > 0 ? ctx->opts.newton_max_iter : 1000;
     k.fp_contract = ctx->opts.fp_contract;
     k.use_pairs = (schedule == 2 && pairs_active(ctx)) ? 1 : 0;
+    // At most one wavefront per SIMD: tell the compiler so.  It may then use
+    // the whole register file and schedule for instruction-level
+    // parallelism, and the dispatcher cannot stack two of the waves on one
+    // SIMD while another idles (measured: 2x on 65536-orbit RK4 at block
+    // sizes of 128 and 512 lanes).
+    const long lanes = (long)ctx->orbits * (schedule == 2 || schedule == 4 ? std::max(group, 1) : 1);
+    k.one_wave_per_simd = (lanes + 63) / 64 <= 4L * device_cus(ctx) ? 1 : 0;
     k.extra_defines = extra;
     return k;
 }
@@ -195,14 +211,6 @@ bool team_possible(const sodecl_ctx* c)
     return c->features.team && (c->solver == SODECL_EULER || c->solver == SODECL_RUNGE_KUTTA);
 }
 
-// Compute units of the context's device (256 on MI355X when it cannot be read).
-int device_cus(const sodecl_ctx* c)
-{
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
-        cus = 256;
-    return cus;
-}
 
 bool group_possible(const sodecl_ctx* c)
 {
@@ -242,8 +250,6 @@ LaunchCfg choose_config(const sodecl_ctx* c)
     int sched = c->opts.schedule;
     if (sched == SODECL_SCHEDULE_AUTO) {
         sched = (group_possible(c) && c->neq >= 8) ? 2 : 1;
-        // fewer orbits than two wavefronts per SIMD in the lane schedule
-        if (sched == 1 && team_possible(c) && (long)c->orbits < 2L * 64 * 4 * device_cus(c)) sched = 4;
     }
     if (sched == SODECL_SCHEDULE_GROUP && !group_possible(c)) sched = 1;
     if (sched == SODECL_SCHEDULE_TEAM && !team_possible(c)) sched = 1;
@@ -271,7 +277,11 @@ LaunchCfg choose_config(const sodecl_ctx* c)
         b = (b / cfg.group) * cfg.group;
         cfg.block = std::min(b, 1024);
     } else {
-        int b = c->opts.block_size > 0 ? c->opts.block_size : (c->local_group_size > 0 ? c->local_group_size : 128);
+        // one wavefront per SIMD or fewer: any block size spreads evenly under
+        // the occupancy hint; otherwise blocks of four wavefronts
+        const bool few = (long)c->orbits <= 64L * 4 * device_cus(c);
+        int b = c->opts.block_size > 0 ? c->opts.block_size
+                                       : (c->local_group_size > 0 ? c->local_group_size : (few ? 128 : 256));
         cfg.block = std::min(std::max(b, 1), 1024);
         cfg.group = 1;
     }

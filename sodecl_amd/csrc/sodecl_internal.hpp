@@ -41,6 +41,7 @@ struct ProgramKey {
     int newton_max_iter = 1000;
     int fp_contract = 0;
     int use_pairs = 0;           // group schedule evaluates the drift through the pairwise form
+    int one_wave_per_simd = 0;   // the launch has at most one wavefront per SIMD (occupancy hint)
     std::string extra_defines;   // additional "#define" lines (adjoint, batch)
 };
 

@@ -197,18 +197,18 @@ def test_pairwise_form_selected_and_lds_bounded():
 
 def test_team_schedule_selection():
     """The team schedule needs the plugin's sode_system_team (oculomotor has
-    it, Kuramoto does not) and an explicit non-stochastic solver; automatic
-    selection takes it while the orbits fill fewer than two wavefronts per
-    SIMD in the lane schedule (256 compute units assumed without a GPU)."""
+    it, Kuramoto does not) and an explicit non-stochastic solver.  It is
+    never chosen automatically (slower than the lane schedule for the
+    oculomotor model: the burst curves are only part of the step)."""
     ocu = sodecl_amd.system_path("oculomotor")
     for solver in (1, 2):
-        s = si.Session(ocu, solver, 65536, 6, 6, 0, 1e-3, 1.0, 10, 0, {})
+        s = si.Session(ocu, solver, 65536, 6, 6, 0, 1e-3, 1.0, 10, 0, {"schedule": "team"})
         assert s.compile_only("gfx950") > 1000
         assert s.launch_config() == (4, 2, 128)
         assert "\nsodecl_team(" in s.program_source()
-    big = si.Session(ocu, 2, 300000, 6, 6, 0, 1e-3, 1.0, 10, 0, {})
-    big.compile_only("gfx950")
-    assert big.launch_config()[0] == 1
+    auto = si.Session(ocu, 2, 65536, 6, 6, 0, 1e-3, 1.0, 10, 0, {})
+    auto.compile_only("gfx950")
+    assert auto.launch_config()[0] == 1
     for solver, nnoi, npar in ((3, 0, 6), (4, 0, 6), (0, 2, 8)):
         s = si.Session(ocu, solver, 300, 6, npar, nnoi, 1e-3, 1.0, 10, 0, {"schedule": "team"})
         s.compile_only("gfx950")
@@ -220,3 +220,26 @@ def test_team_schedule_selection():
                                                             "block_size": 256})
     t4.compile_only("gfx950")
     assert t4.launch_config() == (4, 4, 256)
+
+
+def test_occupancy_hint_and_lane_block():
+    """At most one wavefront per SIMD (1024 SIMDs assumed without a GPU):
+    the kernels carry the one-wave occupancy hint and the lane schedule keeps
+    128-lane blocks; above that, no hint and 256-lane blocks."""
+    ocu = sodecl_amd.system_path("oculomotor")
+    small = si.Session(ocu, 2, 65536, 6, 6, 0, 1e-3, 1.0, 10, 0, {})
+    small.compile_only("gfx950")
+    assert "amdgpu_waves_per_eu(1, 1)" in small.program_source()
+    assert small.launch_config() == (1, 1, 128)
+    big = si.Session(ocu, 2, 65537, 6, 6, 0, 1e-3, 1.0, 10, 0, {})
+    big.compile_only("gfx950")
+    assert "#define SODECL_KERNEL_ATTR __attribute__" not in big.program_source()
+    assert big.launch_config() == (1, 1, 256)
+    # group schedule: lanes = orbits x group size
+    kur = sodecl_amd.system_path("kuramoto")
+    g = si.Session(kur, 2, 16384, 64, 65, 0, 1e-3, 1.0, 10, 0, {"schedule": "group"})
+    g.compile_only("gfx950")
+    assert "#define SODECL_KERNEL_ATTR __attribute__" not in g.program_source()
+    g = si.Session(kur, 1, 1024, 10, 11, 0, 1e-3, 1.0, 10, 0, {"schedule": "group"})
+    g.compile_only("gfx950")
+    assert "#define SODECL_KERNEL_ATTR __attribute__" in g.program_source()
