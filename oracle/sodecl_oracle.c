@@ -581,7 +581,7 @@ long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_co
 /* Check of the division identity the MI355X code and the bundled plugins   */
 /* use for divisions by constants and by orbit parameters: with rd = RN(1/d)*/
 /* q0 = RN(a rd), r = a - q0 d (exact, one fma), q1 = RN(q0 + r rd) equals   */
-/* RN(a / d) (Markstein); copysign/inf handling as in the kernels.  Counts   */
+/* RN(a / d) (Markstein); zeros, inf, NaN pass as in the kernels.  Counts   */
 /* mismatches over n random normal dividends plus special values.           */
 /* ------------------------------------------------------------------------ */
 static double orc_markstein(double a, double d, double rd)
@@ -589,7 +589,7 @@ static double orc_markstein(double a, double d, double rd)
     const double q0 = a * rd;
     const double r = fma(-q0, d, a);
     const double q1 = fma(r, rd, q0);
-    return isfinite(q0) ? copysign(q1, q0) : q0;
+    return (isfinite(q0) && q0 != 0.0) ? q1 : q0;
 }
 
 long orc_markstein_mismatches(double d, long n, uint64_t seed)

@@ -41,16 +41,17 @@ typedef unsigned long long sodecl_u64;
 // (a constant, or hoisted out of the step loop): q0 = a*rd is within one ulp,
 // the remainder a - q0*d is exact in one fused multiply-add, and one more
 // corrects q0 to the correctly rounded quotient (Markstein's theorem), the
-// same bits as IEEE a / d outside the subnormal range.  The correction never
-// changes the sign, so copying q0's sign keeps -0; infinities and NaN pass
-// through.  Three full-rate FP64 operations instead of the quarter-rate
-// reciprocal and scale/fixup sequence of a general division.
+// same bits as IEEE a / d outside the subnormal range.  The correction is
+// applied to finite nonzero q0 only: zeros keep their sign, infinities and
+// NaN pass through (one class test).  Three full-rate FP64 operations instead
+// of the quarter-rate reciprocal and scale/fixup sequence of a general
+// division.
 SODECL_INLINE double sodecl_div(double a, double d, double rd)
 {
     const double q0 = a * rd;
     const double r = fma(-q0, d, a);
     const double q1 = fma(r, rd, q0);
-    return __builtin_isfinite(q0) ? __builtin_copysign(q1, q0) : q0;
+    return (__builtin_isfinite(q0) && q0 != 0.0) ? q1 : q0;
 }
 
 // ---------------------------------------------------------------------------

@@ -34,14 +34,14 @@
 /* a / d given rd = 1/d correctly rounded (d a constant or an orbit
  * parameter, so rd is computed once): q0 = a*rd, exact remainder by one
  * fused multiply-add, one correction (Markstein) -- the bits of IEEE a / d
- * outside the subnormal range, without a general division per call.  The
- * correction cannot change the sign (copysign keeps -0); inf/NaN pass. */
+ * outside the subnormal range, without a general division per call.  Zeros
+ * (which keep their sign), infinities and NaN pass uncorrected. */
 double ocu_div(double a, double d, double rd)
 {
     const double q0 = a * rd;
     const double r = fma(-q0, d, a);
     const double q1 = fma(r, rd, q0);
-    return isfinite(q0) ? copysign(q1, q0) : q0;
+    return (isfinite(q0) && q0 != 0.0) ? q1 : q0;
 }
 
 /* Burst-neuron firing curve: saturating exponential for x > 0, a scaled
@@ -53,11 +53,17 @@ double ocu_div(double a, double d, double rd)
 double ocu_burst(double x, double a, double b, double at, double bt)
 {
     const int pos = x > 0;
-    /* both candidate exponents through the parameters' reciprocals, which
-     * depend on the orbit only and so are computed once per run */
-    const double up = ocu_div(-x, bt, 1.0 / bt);
-    const double dn = ocu_div(x, b, 1.0 / b);
-    const double e = exp(pos ? up : dn);
+    /* one division through the selected parameter's reciprocal; both
+     * reciprocals depend on the orbit only and are computed once per run.
+     * (2/bt)*0.5 is exactly 1/bt (scaling by powers of two commutes with
+     * rounding); written so, the two candidates are not the same operation
+     * on different operands, which a compiler would turn into one division
+     * of the selected denominator inside the step loop. */
+    const double rbt = (2.0 / bt) * 0.5;
+    const double rb = 1.0 / b;
+    /* the numerator -x (x > 0) or x (x <= 0) is -|x|, a free source
+     * modifier; for x = +0 it is -0, whose quotient's exp is 1 as well */
+    const double e = exp(ocu_div(-fabs(x), pos ? bt : b, pos ? rbt : rb));
     return pos ? at * (1.00 - e) : -(a / b) * x * e;
 }
 

@@ -220,6 +220,15 @@ def test_plugin_matches_reference_example(solver):
     a = RefKernels("oculomotor", solver).run(y0, p, 1e-6, 2e-3, 40)
     b = RefKernels("refoculomotor", solver).run(y0, p, 1e-6, 2e-3, 40)
     np.testing.assert_array_equal(a["out"], b["out"])
+    # randomised parameters and states: both signs of the burst input, the
+    # benchmark's parameter spread, blow-ups included (NaN compared as equal)
+    from sodecl_amd import configs
+    cfg = configs.oculomotor_rk4(orbits=96, seed=11, dt=1e-4, tspan=0.05, ksteps=25)
+    y0 = cfg["y0"].copy()
+    y0[:, 5] = np.linspace(-2.0, 2.0, 96)
+    a = RefKernels("oculomotor", solver).run(y0, cfg["params"], 1e-4, 0.05, 25)
+    b = RefKernels("refoculomotor", solver).run(y0, cfg["params"], 1e-4, 0.05, 25)
+    np.testing.assert_array_equal(a["out"].view(np.int64), b["out"].view(np.int64))
 
 
 @pytest.mark.parametrize("d", [6.0, 25.0, 0.15, 9.0, 1.5, 0.0045 * 1.7])
