@@ -83,6 +83,8 @@ typedef struct sodecl_options {
     int pairwise;            /* group schedule with a plugin that defines the pairwise drift (sode_pair,
                                 sode_system_pairsum): 0 = automatic (used while the per-orbit N x (N+1)
                                 LDS matrix is at most 16 KiB), 1 = always, -1 = never */
+    double newton_tol;       /* Newton stop threshold of the implicit solvers (_epsilon1_); 0 = 1e-07
+                                (src/sodeclmgr.cpp:424-430) */
 } sodecl_options;
 
 typedef struct sodecl_ctx sodecl_ctx;
@@ -163,10 +165,15 @@ int sodecl_run_device(sodecl_ctx* ctx, double* t0, double* y0, const double* par
                       double* traj, void* stream);
 
 /* ---- Discrete adjoint (no reference counterpart) ----------------------
- * For the Euler and RK4 solvers under the group schedule, with a plugin
- * that defines sode_system_i and sode_system_vjp_group.  Given dL/dtraj for
- * a trajectory of the context's configuration, returns dL/dy0[orbits][neq]
- * and dL/dparams[orbits][npar], the exact transpose of the forward steps.
+ * Euler and RK4 under the group schedule, with a plugin that defines
+ * sode_system_i and sode_system_vjp_group: the exact transpose of the
+ * forward steps.  Implicit Euler and implicit midpoint under the lane
+ * schedule, with a plugin that defines calc_jacobian (or calc_jacobian_exact,
+ * read instead when present) and calc_jacobian_params: the derivative of
+ * each step's implicit equation at
+ * its converged Newton solution (the transposed Newton system solved by
+ * tridiagonal sweeps).  Given dL/dtraj for a trajectory of the context's
+ * configuration, returns dL/dy0[orbits][neq] and dL/dparams[orbits][npar].
  * Memory is bounded by checkpointing every ckpt_every steps
  * (sodecl_default_checkpoint_interval: ceil(sqrt(steps))). */
 int sodecl_default_checkpoint_interval(const sodecl_ctx* ctx);

@@ -42,6 +42,7 @@ class _Config(ctypes.Structure):
         ("nthreads", ctypes.c_int),
         ("nlaunch", ctypes.c_long),
         ("use_band", ctypes.c_int),
+        ("newton_tol", ctypes.c_double),
     ]
 
 
@@ -77,19 +78,23 @@ class Oracle:
         L.orc_sensitivity_gradient.restype = ctypes.c_long
         L.orc_sensitivity_gradient.argtypes = [ctypes.POINTER(_Config), ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                                _dp, _dp]
+        L.orc_jacobians.restype = ctypes.c_long
+        L.orc_jacobians.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
         L.orc_counter_word.restype = ctypes.c_uint64
         L.orc_counter_word.argtypes = [ctypes.c_double]
 
     def run(self, solver, y0, params, dt, tspan, ksteps, t0=None, counters=None, nnoi=0,
-            newton_maxit=1000, orbit_range=None, nthreads=0, want_output=True, nlaunch=-1, use_band=False):
+            newton_maxit=1000, orbit_range=None, nthreads=0, want_output=True, nlaunch=-1, use_band=False,
+            newton_tol=0.0):
         """Integrates every orbit; returns dict(out, y, t, counters, newton_updates, launches).
-        nlaunch >= 0 fixes the number of ksteps segments instead of tspan."""
+        nlaunch >= 0 fixes the number of ksteps segments instead of tspan;
+        newton_tol > 0 replaces the reference's Newton threshold 1e-07."""
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
         params = np.ascontiguousarray(params, dtype=np.float64)
         orbits, neq = y0.shape
         npar = params.shape[1]
         c = _Config(SOLVERS[solver] if isinstance(solver, str) else int(solver), neq, npar, nnoi,
-                    orbits, ksteps, dt, tspan, newton_maxit, nthreads, nlaunch, int(use_band))
+                    orbits, ksteps, dt, tspan, newton_maxit, nthreads, nlaunch, int(use_band), float(newton_tol))
         ob, oc = (0, orbits) if orbit_range is None else orbit_range
         t = np.zeros(orbits) if t0 is None else np.array(t0, dtype=np.float64)
         y = y0.copy()
@@ -103,24 +108,36 @@ class Oracle:
             raise ValueError("oracle rejected the configuration")
         return dict(out=out, y=y, t=t, counters=cnt, newton_updates=nu, launches=int(r))
 
-    def sensitivity_gradient(self, solver, y0, params, cot, dt, tspan, ksteps, orbit_range=None, nthreads=0):
+    def sensitivity_gradient(self, solver, y0, params, cot, dt, tspan, ksteps, orbit_range=None, nthreads=0,
+                             newton_tol=0.0):
         """Gradient of sum <cot[snap], y(snap)> w.r.t. y0 and params by forward
         sensitivity; cot is [nsnap][neq][orbits].  Returns (grad_y0, grad_p)
-        for the orbit range."""
+        for the orbit range.  Implicit solvers: tangent of each step's
+        implicit equation at its converged Newton solution."""
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
         params = np.ascontiguousarray(params, dtype=np.float64)
         cot = np.ascontiguousarray(cot, dtype=np.float64)
         orbits, neq = y0.shape
         npar = params.shape[1]
-        c = _Config(SOLVERS[solver], neq, npar, 0, orbits, ksteps, dt, tspan, 1000, nthreads, -1, 0)
+        c = _Config(SOLVERS[solver], neq, npar, 0, orbits, ksteps, dt, tspan, 1000, nthreads, -1, 0, float(newton_tol))
         ob, oc = (0, orbits) if orbit_range is None else orbit_range
         gy = np.zeros((oc, neq))
         gp = np.zeros((oc, npar))
         r = self.lib.orc_sensitivity_gradient(ctypes.byref(c), ob, oc, _ptr(y0), _ptr(params), _ptr(cot), _ptr(gy),
                                               _ptr(gp))
         if r < 0:
-            raise ValueError("sensitivity needs calc_jacobian/calc_jacobian_params and an explicit solver")
+            raise ValueError("sensitivity needs calc_jacobian/calc_jacobian_params and a deterministic solver")
         return gy, gp
+
+    def jacobians(self, y, p, exact=True):
+        """The plugin's (df/dy [neq][neq], df/dp [neq][npar]) at (y, p)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        jac = np.zeros((y.size, y.size))
+        jp = np.zeros((y.size, max(p.size, 1)))
+        if self.lib.orc_jacobians(y.size, p.size, _ptr(y), _ptr(p), _ptr(jac), _ptr(jp), int(exact)) < 0:
+            raise ValueError("the plugin lacks calc_jacobian / calc_jacobian_params")
+        return jac, jp[:, :p.size]
 
     def threefry(self, ctr, key):
         c = (ctypes.c_uint64 * 2)(*ctr)

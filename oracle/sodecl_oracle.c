@@ -69,6 +69,7 @@ void sode_system_stoch(double t, double* y, double* stoch, double* p, double* no
 void calc_jacobian(double* y, double (*jac)[orc_numeq], double* p) __attribute__((weak));
 void calc_jacobian_params(double* y, double (*jp)[orc_numpar], double* p) __attribute__((weak));
 void calc_jacobian_band(double* y, double* lower, double* diag, double* upper, double* p) __attribute__((weak));
+void calc_jacobian_exact(double* y, double (*jac)[orc_numeq], double* p) __attribute__((weak));
 
 /* 1: build the Newton matrix from calc_jacobian_band when the plugin has it
  * (the same three diagonals the tridiagonal solve reads from the dense
@@ -323,6 +324,7 @@ typedef struct {
     int nthreads;
     long nlaunch;      /* >= 0: number of ksteps segments, overriding tspan */
     int use_band;      /* build Newton matrices from calc_jacobian_band */
+    double newton_tol; /* > 0: Newton stop threshold in place of _epsilon1_ = 1e-07 */
 } orc_config;
 
 /* Integrates one orbit for one segment of ksteps steps, exactly as one
@@ -387,6 +389,7 @@ static void orc_set_globals(const orc_config* c)
     orc_kdt = orc_kernel_dt(c->dt);
     orc_newton_maxit = c->newton_maxit > 0 ? c->newton_maxit : 1000;
     orc_use_band = c->use_band;
+    orc_epsilon1 = c->newton_tol > 0 ? c->newton_tol : 1e-07;
 }
 
 /*
@@ -494,20 +497,114 @@ static void orc_tangent_rhs(double t, double* x, double* p, const double* dx, do
     free(jp);
 }
 
+/* The plugin's Jacobians at (y, p) for the checks of tests: jac [neq][neq]
+ * from calc_jacobian_exact when exact and the plugin has it, else
+ * calc_jacobian; jp [neq][npar] from calc_jacobian_params.  Returns -1 when
+ * an entry point is missing. */
+long orc_jacobians(int neq, int npar, double* y, double* p, double* jac, double* jp, int exact)
+{
+    orc_numeq = neq;
+    orc_numpar = npar;
+    if (!calc_jacobian || (npar > 0 && !calc_jacobian_params)) return -1;
+    if (exact && calc_jacobian_exact) calc_jacobian_exact(y, (double (*)[orc_numeq])jac, p);
+    else calc_jacobian(y, (double (*)[orc_numeq])jac, p);
+    if (npar > 0) calc_jacobian_params(y, (double (*)[orc_numpar])jp, p);
+    return 0;
+}
+
+/* Solves A X = B in place (A n x n row-major, B n x ncol row-major) by
+ * Gaussian elimination with partial pivoting; X replaces B. */
+static void orc_dense_solve(int n, double* A, double* B, int ncol)
+{
+    for (int k = 0; k < n; k++) {
+        int piv = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(A[(size_t)i * n + k]) > fabs(A[(size_t)piv * n + k])) piv = i;
+        if (piv != k) {
+            for (int j = 0; j < n; j++) {
+                double tmp = A[(size_t)k * n + j];
+                A[(size_t)k * n + j] = A[(size_t)piv * n + j];
+                A[(size_t)piv * n + j] = tmp;
+            }
+            for (int j = 0; j < ncol; j++) {
+                double tmp = B[(size_t)k * ncol + j];
+                B[(size_t)k * ncol + j] = B[(size_t)piv * ncol + j];
+                B[(size_t)piv * ncol + j] = tmp;
+            }
+        }
+        for (int i = k + 1; i < n; i++) {
+            double f = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+            if (f == 0) continue;
+            for (int j = k; j < n; j++) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+            for (int j = 0; j < ncol; j++) B[(size_t)i * ncol + j] -= f * B[(size_t)k * ncol + j];
+        }
+    }
+    for (int k = n - 1; k >= 0; k--)
+        for (int j = 0; j < ncol; j++) {
+            double a = B[(size_t)k * ncol + j];
+            for (int m = k + 1; m < n; m++) a -= A[(size_t)k * n + m] * B[(size_t)m * ncol + j];
+            B[(size_t)k * ncol + j] = a / A[(size_t)k * n + k];
+        }
+}
+
+/* Tangent of one implicit step (IE: c = 1, x = g; IM: c = 1/2,
+ * x = (y + g)/2) by the implicit function theorem at the converged Newton
+ * solution g:
+ *   IE: (I - dt J(g)) S' = S + dt [0 | f_p(g)]
+ *   IM: (I - dt/2 J(m)) S' = (I + dt/2 J(m)) S + dt [0 | f_p(m)]
+ * with J the plugin's exact df/dy (calc_jacobian_exact, else calc_jacobian).
+ * S is n x ncol (ncol = n + npar), updated in place. */
+static void orc_implicit_tangent(int solver, double dt, const double* y, const double* g, double* p, double* S,
+                                 int ncol)
+{
+    int n = _numeq_, np = _numpar_;
+    double c = solver == ORC_IE ? 1.0 : 0.5;
+    double x[n];
+    for (int i = 0; i < n; i++) x[i] = solver == ORC_IE ? g[i] : (y[i] + g[i]) * 0.5;
+    double (*jac)[n] = malloc(sizeof(double) * (size_t)n * (size_t)n);
+    double (*jp)[np > 0 ? np : 1] = malloc(sizeof(double) * (size_t)n * (size_t)(np > 0 ? np : 1));
+    if (calc_jacobian_exact) calc_jacobian_exact(x, jac, p);
+    else calc_jacobian(x, jac, p);
+    if (np > 0) calc_jacobian_params(x, jp, p);
+    double* A = malloc(sizeof(double) * (size_t)n * (size_t)n);
+    double* B = malloc(sizeof(double) * (size_t)n * (size_t)ncol);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) A[(size_t)i * n + j] = (i == j ? 1.0 : 0.0) - c * dt * jac[i][j];
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < ncol; k++) {
+            double b = S[(size_t)i * ncol + k];
+            if (solver == ORC_IM) {
+                double a = 0;
+                for (int m = 0; m < n; m++) a += jac[i][m] * S[(size_t)m * ncol + k];
+                b += 0.5 * dt * a;
+            }
+            if (k >= n) b += dt * jp[i][k - n];
+            B[(size_t)i * ncol + k] = b;
+        }
+    orc_dense_solve(n, A, B, ncol);
+    memcpy(S, B, sizeof(double) * (size_t)n * (size_t)ncol);
+    free(A);
+    free(B);
+    free(jac);
+    free(jp);
+}
+
 /*
  * Gradient of L = sum_snap <cot[snap], y(snap)> with respect to y0 and the
  * parameters of each orbit in [orbit_begin, orbit_begin + orbit_count), by
  * propagating S = d y / d(y0, p) through the same discrete steps as the
- * forward integration (Euler or RK4).  cot is [nsnap][neq][orbits] over all
- * orbits; grad_y0 [count][neq], grad_p [count][npar].  Returns 0, or -1 when
- * the plugin lacks calc_jacobian / calc_jacobian_params or the solver is
- * not explicit.
+ * forward integration: the exact tangent of the Euler or RK4 step, or for
+ * implicit Euler / midpoint the tangent of each step's implicit equation at
+ * the converged Newton solution (orc_implicit_tangent).  cot is
+ * [nsnap][neq][orbits] over all orbits; grad_y0 [count][neq], grad_p
+ * [count][npar].  Returns 0, or -1 when the plugin lacks calc_jacobian /
+ * calc_jacobian_params or the solver is stochastic.
  */
 long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_count, const double* y0,
                               const double* params, const double* cot, double* grad_y0, double* grad_p)
 {
     if (!calc_jacobian || (c->npar > 0 && !calc_jacobian_params)) return -1;
-    if (c->solver != ORC_EULER && c->solver != ORC_RK4) return -1;
+    if (c->solver == ORC_SEULER || c->solver < 0 || c->solver > 4) return -1;
     orc_set_globals(c);
     long launches = c->nlaunch >= 0 ? c->nlaunch : orc_num_launches(c->tspan, c->dt, c->ksteps);
     int n = c->neq, np = c->npar, ncol = n + np;
@@ -541,7 +638,13 @@ long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_co
                 }
             }
             if (s == total) break;
-            if (c->solver == ORC_EULER) {
+            if (c->solver == ORC_IE || c->solver == ORC_IM) {
+                double g[n];
+                if (c->solver == ORC_IE) orc_ie(dt, t, y, g, pp);
+                else orc_im(dt, t, y, g, pp);
+                orc_implicit_tangent(c->solver, dt, y, g, pp, S, ncol);
+                for (int i = 0; i < n; i++) y[i] = g[i];
+            } else if (c->solver == ORC_EULER) {
                 orc_tangent_rhs(t, y, pp, S, dk, ncol);
                 sode_system(t, y, f, pp);
                 for (size_t q = 0; q < (size_t)n * ncol; q++) S[q] = S[q] + dk[q] * dt;

@@ -18,13 +18,27 @@ int nstage(const sodecl_ctx* c) { return c->solver == SODECL_RUNGE_KUTTA ? 4 : 1
 
 long total_steps(const sodecl_ctx* c) { return num_launches(c->tspan, c->dt, c->ksteps) * (long)c->ksteps; }
 
+bool implicit(const sodecl_ctx* c)
+{
+    return c->solver == SODECL_IMPLICIT_EULER || c->solver == SODECL_IMPLICIT_MIDPOINT;
+}
+
+// Euler/RK4: group schedule with the plugin's group VJP.  IE/IM: lane
+// schedule with the plugin's dense Jacobians (calc_jacobian,
+// calc_jacobian_params).
 int check_adjoint(sodecl_ctx* c, int& sched, int& group, int& block)
 {
+    launch_config(c, &sched, &group, &block);
+    if (implicit(c)) {
+        if (!c->features.calc_jacobian || (c->npar > 0 && !c->features.jacobian_params))
+            return fail(c, "the implicit adjoint needs calc_jacobian and calc_jacobian_params in the system source");
+        if (sched != SODECL_SCHEDULE_LANE) return fail(c, "the implicit adjoint runs under the lane schedule");
+        return 1;
+    }
     if (c->solver != SODECL_EULER && c->solver != SODECL_RUNGE_KUTTA)
-        return fail(c, "the adjoint supports the Euler and RK4 solvers");
+        return fail(c, "the adjoint supports the Euler, RK4, implicit Euler and implicit midpoint solvers");
     if (!c->features.component || !c->features.vjp_group)
         return fail(c, "the adjoint needs sode_system_i and sode_system_vjp_group in the system source");
-    launch_config(c, &sched, &group, &block);
     if (sched != SODECL_SCHEDULE_GROUP)
         return fail(c, "the adjoint runs under the group schedule (set schedule to group)");
     return 1;
@@ -54,6 +68,8 @@ size_t sodecl_adjoint_workspace_size(const sodecl_ctx* c, int ckpt_every)
     if (!c || ckpt_every <= 0) return 0;
     int sched = 0, group = 0, block = 0;
     launch_config(c, &sched, &group, &block);
+    if (implicit(c))   // rows 0..ckpt_every of [neq + 1][orbits]
+        return sched == SODECL_SCHEDULE_LANE ? (size_t)c->orbits * (size_t)(ckpt_every + 1) * (size_t)(c->neq + 1) : 0;
     if (sched != SODECL_SCHEDULE_GROUP) return 0;
     const int opb = block / group;
     const size_t padded = (size_t)((c->orbits + opb - 1) / opb) * opb;
@@ -82,12 +98,13 @@ int sodecl_adjoint_device(sodecl_ctx* c, const double* ckpt_t, const double* ckp
     if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
     auto prog = get_program(c, sched, group, block, kAdjointDefines);
     if (!prog) return 0;
-    hipFunction_t fn = prog->get("sodecl_group_adjoint");
-    if (!fn) return fail(c, "kernel sodecl_group_adjoint missing from the program");
+    const char* kname = implicit(c) ? "sodecl_lane_adjoint" : "sodecl_group_adjoint";
+    hipFunction_t fn = prog->get(kname);
+    if (!fn) return fail(c, std::string("kernel ") + kname + " missing from the program");
     const long steps = total_steps(c);
     if (steps > 2147483647L) return fail(c, "too many steps");
     int orbits = c->orbits, nsteps = (int)steps, every = ckpt_every;
-    const int opb = block / group;
+    const int opb = implicit(c) ? block : block / group;
     const unsigned grid = (unsigned)((orbits + opb - 1) / opb);
     void* args[] = {(void*)&ckpt_t, (void*)&ckpt_y, (void*)&params, (void*)&cot, &grad_y0, &grad_params,
                     &work, &orbits, &nsteps, &every};

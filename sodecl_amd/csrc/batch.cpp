@@ -77,6 +77,7 @@ ProgramKey batch_key(const sodecl_batch* b)
     k.ksteps = b->ksteps;
     k.dt = b->dt;
     k.newton_max_iter = b->opts.newton_max_iter > 0 ? b->opts.newton_max_iter : 1000;
+    k.newton_tol = b->opts.newton_tol;
     k.fp_contract = b->opts.fp_contract;
     return k;
 }

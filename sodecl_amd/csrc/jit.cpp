@@ -78,6 +78,8 @@ PluginFeatures detect_features(const std::string& source)
     f.vjp_group = defines("sode_system_vjp_group");
     f.pairs = defines("sode_pair") && defines("sode_system_pairsum");
     f.team = defines("sode_system_team");
+    f.jacobian_params = defines("calc_jacobian_params");
+    f.jacobian_exact = defines("calc_jacobian_exact");
     return f;
 }
 
@@ -103,6 +105,16 @@ long num_launches(double tspan, double dt, int ksteps)
     return n;
 }
 
+// "#define _epsilon1_ 1e-07" as the reference prints it (sodeclmgr.cpp:424-430),
+// or the caller's Newton threshold written exactly.
+std::string epsilon_define(const ProgramKey& k)
+{
+    if (!(k.newton_tol > 0)) return "#define _epsilon1_ 1e-07\n";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "#define _epsilon1_ %.17g\n", k.newton_tol);
+    return buf;
+}
+
 std::string assemble_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f)
 {
     std::ostringstream o;
@@ -114,7 +126,7 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     o << "#define _m_dt_ " << dt_literal(k.dt) << "\n";
     o << "#define _numeq_ " << k.neq << "\n";
     o << "#define _numpar_ " << k.npar << "\n";
-    if (k.solver == SODECL_IMPLICIT_EULER || k.solver == SODECL_IMPLICIT_MIDPOINT) o << "#define _epsilon1_ 1e-07\n";
+    if (k.solver == SODECL_IMPLICIT_EULER || k.solver == SODECL_IMPLICIT_MIDPOINT) o << epsilon_define(k);
     // MI355X build configuration
     o << "#define SODECL_HIP 1\n";
     o << "#define SODECL_SOLVER " << k.solver << "\n";
@@ -128,6 +140,8 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     o << "#define SODECL_HAS_JACOBIAN_BAND_I " << (f.jacobian_band_i ? 1 : 0) << "\n";
     o << "#define SODECL_HAS_JACOBIAN " << (f.calc_jacobian ? 1 : 0) << "\n";
     o << "#define SODECL_HAS_VJP_GROUP " << (f.vjp_group ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN_PARAMS " << (f.jacobian_params ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN_EXACT " << (f.jacobian_exact ? 1 : 0) << "\n";
     o << "#define SODECL_USE_PAIRS " << (k.schedule == 2 && k.use_pairs && f.pairs ? 1 : 0) << "\n";
     if (k.one_wave_per_simd) o << "#define SODECL_KERNEL_ATTR __attribute__((amdgpu_waves_per_eu(1, 1)))\n";
     o << k.extra_defines;
@@ -160,7 +174,7 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
     o << "#define _numsteps_ " << k.ksteps << "\n";
     o << "#define _m_dt_ " << dt_literal(k.dt) << "\n";
     o << "#define _numpar_ " << k.npar << "\n";
-    o << "#define _epsilon1_ 1e-07\n";
+    o << epsilon_define(k);
     o << "#define SODECL_HIP 1\n";
     o << "#define SODECL_SOLVER " << k.solver << "\n";
     o << "#define SODECL_SCHEDULE 2\n";

@@ -82,6 +82,7 @@ ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, c
     k.group = group;
     k.block = block;
     k.newton_max_iter = ctx->opts.newton_max_iter > 0 ? ctx->opts.newton_max_iter : 1000;
+    k.newton_tol = ctx->opts.newton_tol;
     k.fp_contract = ctx->opts.fp_contract;
     k.use_pairs = (schedule == 2 && pairs_active(ctx)) ? 1 : 0;
     // At most one wavefront per SIMD: tell the compiler so.  It may then use
@@ -847,7 +848,10 @@ int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
     if (o->schedule < 0 || o->schedule > SODECL_SCHEDULE_TEAM) return fail(c, "invalid schedule");
     if (o->group_size < 0 || o->group_size > 64) return fail(c, "group_size must be in 0..64");
     if (o->block_size < 0 || o->block_size > 1024) return fail(c, "block_size must be in 0..1024");
-    bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract;
+    if (!(o->newton_tol >= 0)) return fail(c, "newton_tol must be non-negative");
+    // options that change the program text
+    bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract ||
+                     o->pairwise != c->opts.pairwise || o->newton_tol != c->opts.newton_tol;
     c->opts = *o;
     if (recompile && c->ready) {
         for (auto& kv : c->programs)
@@ -990,6 +994,9 @@ size_t sodecl_compile_only(sodecl_ctx* c, const char* arch)
     std::vector<std::string> variants = {std::string()};
     // the adjoint program too, where this configuration supports one
     if (cfg.schedule == 2 && c->features.vjp_group && (c->solver == SODECL_EULER || c->solver == SODECL_RUNGE_KUTTA))
+        variants.push_back("#define SODECL_ADJOINT 1\n");
+    if (cfg.schedule == 1 && c->features.calc_jacobian && (c->npar == 0 || c->features.jacobian_params) &&
+        (c->solver == SODECL_IMPLICIT_EULER || c->solver == SODECL_IMPLICIT_MIDPOINT))
         variants.push_back("#define SODECL_ADJOINT 1\n");
     for (auto& extra : variants) {
         key.extra_defines = extra;

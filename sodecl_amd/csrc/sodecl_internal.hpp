@@ -25,6 +25,8 @@ struct PluginFeatures {
     bool vjp_group = false;         // sode_system_vjp_group(...) (adjoint, group schedule)
     bool pairs = false;             // sode_pair + sode_system_pairsum (antisymmetric pairwise drift)
     bool team = false;              // sode_system_team(...) (team schedule)
+    bool jacobian_params = false;   // calc_jacobian_params(y, jp[neq][npar], p) (implicit adjoint)
+    bool jacobian_exact = false;    // calc_jacobian_exact(y, jac[neq][neq], p) (implicit adjoint)
 };
 
 PluginFeatures detect_features(const std::string& source);
@@ -39,6 +41,7 @@ struct ProgramKey {
     int group = 64;      // lanes per orbit (group schedule)
     int block = 128;
     int newton_max_iter = 1000;
+    double newton_tol = 0.0;     // > 0: _epsilon1_ in place of the reference's 1e-07
     int fp_contract = 0;
     int use_pairs = 0;           // group schedule evaluates the drift through the pairwise form
     int one_wave_per_simd = 0;   // the launch has at most one wavefront per SIMD (occupancy hint)

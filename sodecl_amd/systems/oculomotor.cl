@@ -24,7 +24,9 @@
  * implicit solvers, returns only the three diagonals the tridiagonal solve
  * uses (lower[i] = J[i+1][i], diag[i] = J[i][i], upper[i] = J[i][i+1]);
  * sode_system_team, read by the team schedule, shares the two burst curves
- * between the two lanes of a pair.
+ * between the two lanes of a pair; calc_jacobian_exact and
+ * calc_jacobian_params (df/dy and df/dp) are read by the discrete adjoint of
+ * the implicit solvers.
  */
 
 #define OCU_T1 0.15
@@ -177,4 +179,67 @@ void calc_jacobian_band(double y[_numeq_], double lower[_numeq_], double diag[_n
     lower[3] = -(2 * p[3] * y[4] * y[3]) / p[2];
     lower[4] = 1;
     lower[5] = 0;
+}
+
+/* The true df/dy, read by the discrete adjoint of the implicit solvers.  It
+ * is calc_jacobian except J[4][5]: yout[4] depends on y5 through B(-y5), so
+ * dyout[4]/dy5 = -B'(-y5)/p2.  calc_jacobian keeps the reference's entry
+ * (examples/oculomotor.cl:75, without the minus sign) because the Newton
+ * iteration of the implicit solvers uses it as its matrix. */
+void calc_jacobian_exact(double y[_numeq_], double jac[_numeq_][_numeq_], double p[_numpar_])
+{
+    calc_jacobian(y, jac, p);
+    jac[4][5] = -jac[4][5];
+}
+
+/* Partial derivatives of the burst curve B(x; a, b, at, bt) with respect to
+ * (a, b, at, bt). */
+void ocu_burst_dp(double x, double a, double b, double at, double bt, double d[4])
+{
+    if (x > 0)
+    {
+        const double e = exp(-x / bt);
+        d[0] = 0;
+        d[1] = 0;
+        d[2] = 1.00 - e;
+        d[3] = -(at * e * x) / (bt * bt);
+    }
+    else
+    {
+        const double e = exp(x / b);
+        d[0] = -(x / b) * e;
+        d[1] = ((a * x * e) / (b * b)) * (1.00 + x / b);
+        d[2] = 0;
+        d[3] = 0;
+    }
+}
+
+/* df/dp (jp[i][k] = dyout[i]/dp[k]): only yout[3] and yout[4] depend on the
+ * parameters; the noise amplitudes of the SDE form do not enter the drift. */
+void calc_jacobian_params(double y[_numeq_], double jp[_numeq_][_numpar_], double p[_numpar_])
+{
+    for (int i = 0; i < _numeq_; i++)
+    {
+        for (int k = 0; k < _numpar_; k++)
+        {
+            jp[i][k] = 0;
+        }
+    }
+    double d3[4], d4[4];
+    ocu_burst_dp(y[5], p[0], p[1], p[4], p[5], d3);
+    ocu_burst_dp(-y[5], p[0], p[1], p[4], p[5], d4);
+    const double s3 = -y[3] - (p[3] * y[3] * y[4] * y[4]) + ocu_burst(y[5], p[0], p[1], p[4], p[5]);
+    const double s4 = -y[4] - (p[3] * y[4] * y[3] * y[3]) + ocu_burst(-y[5], p[0], p[1], p[4], p[5]);
+    jp[3][0] = d3[0] / p[2];
+    jp[3][1] = d3[1] / p[2];
+    jp[3][2] = -s3 / (p[2] * p[2]);
+    jp[3][3] = -(y[3] * y[4] * y[4]) / p[2];
+    jp[3][4] = d3[2] / p[2];
+    jp[3][5] = d3[3] / p[2];
+    jp[4][0] = d4[0] / p[2];
+    jp[4][1] = d4[1] / p[2];
+    jp[4][2] = -s4 / (p[2] * p[2]);
+    jp[4][3] = -(y[4] * y[3] * y[3]) / p[2];
+    jp[4][4] = d4[2] / p[2];
+    jp[4][5] = d4[3] / p[2];
 }

@@ -8,9 +8,10 @@
 The forward pass runs the MI355X integrator (libsodecl_amd.so) on the
 tensors' memory and writes a checkpoint every ckpt_every steps (default
 ceil(sqrt(steps))); backward recomputes each checkpoint segment and sweeps
-it backwards with the exact transpose of the Euler / RK4 step
-(sodecl_adjoint_device).  PyTorch provides only device memory and the
-stream; all arithmetic is in the HIP kernels.
+it backwards (sodecl_adjoint_device): the exact transpose of the Euler /
+RK4 step, or for implicit Euler / midpoint the derivative of each step's
+implicit equation at its converged Newton solution.  PyTorch provides only
+device memory and the stream; all arithmetic is in the HIP kernels.
 """
 import os
 

@@ -126,3 +126,90 @@ def test_adjoint_vs_central_differences_sincos_loss():
             fd = (lp - lm) / (2 * h)
             an = (gp if kind == "p" else gy)[o, idx]
             assert abs(an - fd) <= 1e-6 * max(abs(fd), 1e-3 * float(loss)), (kind, o, idx, an, fd)
+
+
+def make_implicit(system, solver, orbits, seed, tol=1e-14, ckpt_every=0):
+    rng = np.random.default_rng(seed)
+    if system == "oculomotor":
+        n, npar = 6, 6
+        cfg = configs.oculomotor_rk4(orbits=orbits, seed=seed, dt=1e-4, tspan=0.02, ksteps=20)
+        y0 = cfg["y0"] + rng.normal(0, 0.05, (orbits, n))
+        y0[:, 5] += 1.0
+        p = cfg["params"]
+    else:
+        n, npar = 5, 6
+        cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=seed, dt=0.01, tspan=1.0, ksteps=10)
+        y0, p = cfg["y0"], cfg["params"]
+    integ = Integrator(cfg["system"], solver, orbits, n, npar, cfg["dt"], cfg["tspan"], cfg["ksteps"],
+                       ckpt_every=ckpt_every, newton_tol=tol)
+    return cfg, integ, y0, p
+
+
+@pytest.mark.parametrize("system", ["oculomotor", "kuramoto"])
+@pytest.mark.parametrize("solver", [3, 4])
+def test_implicit_adjoint_vs_forward_sensitivity(system, solver):
+    """Implicit Euler / midpoint adjoint (transposed Newton system at the
+    converged solution, tridiagonal sweeps) against the oracle's forward
+    sensitivity (dense solve of the tangent system), both with the Newton
+    threshold at 1e-14 so that the two integrations converge to the same
+    states; 1e-9 relative to the largest gradient entry."""
+    orbits = 29
+    cfg, integ, y0, p = make_implicit(system, solver, orbits, seed=solver)
+    yt = torch.tensor(y0, device=DEV, requires_grad=True)
+    pt = torch.tensor(p, device=DEV, requires_grad=True)
+    rng = np.random.default_rng(solver + 10)
+    cot_np = rng.normal(size=(integ.nsnap, integ.neq, orbits))
+    gy, gp = grads(integ, yt, pt, torch.tensor(cot_np, device=DEV))
+    name = {3: "ie", 4: "im"}[solver]
+    ry, rp = Oracle(system).sensitivity_gradient(name, y0, p, cot_np, cfg["dt"], cfg["tspan"], cfg["ksteps"],
+                                                 newton_tol=1e-14)
+    assert rel(gy, ry) < 1e-9, rel(gy, ry)
+    assert rel(gp, rp) < 1e-9, rel(gp, rp)
+
+
+@pytest.mark.parametrize("system,solver", [("oculomotor", 3), ("kuramoto", 4)])
+def test_implicit_adjoint_vs_central_differences(system, solver):
+    orbits = 7
+    cfg, integ, y0, p = make_implicit(system, solver, orbits, seed=2)
+    rng = np.random.default_rng(7)
+    cot_np = rng.normal(size=(integ.nsnap, integ.neq, orbits))
+    cot = torch.tensor(cot_np, device=DEV)
+    yt = torch.tensor(y0, device=DEV, requires_grad=True)
+    pt = torch.tensor(p, device=DEV, requires_grad=True)
+    gy, gp = grads(integ, yt, pt, cot)
+
+    def L(yv, pv):
+        with torch.no_grad():
+            traj = integ(torch.tensor(yv, device=DEV), torch.tensor(pv, device=DEV))
+            return (traj * cot).sum(dim=(0, 1)).cpu().numpy()
+
+    for j in range(integ.neq):
+        h = 1e-6
+        yp, ym = y0.copy(), y0.copy()
+        yp[:, j] += h
+        ym[:, j] -= h
+        fd = (L(yp, p) - L(ym, p)) / (2 * h)
+        assert np.max(np.abs(gy[:, j] - fd)) <= 1e-6 * max(np.abs(gy).max(), 1e-300), (j, gy[:, j], fd)
+    for k in range(integ.npar):
+        h = 1e-6 * np.abs(p[:, k]).max()
+        pp, pm = p.copy(), p.copy()
+        pp[:, k] += h
+        pm[:, k] -= h
+        fd = (L(y0, pp) - L(y0, pm)) / (2 * h)
+        assert np.max(np.abs(gp[:, k] - fd)) <= 1e-6 * max(np.abs(gp).max(), 1e-300), (k, gp[:, k], fd)
+
+
+def test_implicit_adjoint_checkpoint_interval_and_default_threshold():
+    """Gradients are bit-identical for every checkpoint interval (each
+    segment is recomputed with the forward's own Newton step), also at the
+    reference's Newton threshold 1e-07."""
+    orbits = 33
+    res = []
+    for every in (0, 1, 7, 1000):
+        cfg, integ, y0, p = make_implicit("kuramoto", 3, orbits, seed=4, tol=0.0, ckpt_every=every)
+        yt = torch.tensor(y0, device=DEV, requires_grad=True)
+        pt = torch.tensor(p, device=DEV, requires_grad=True)
+        cot = torch.ones((integ.nsnap, integ.neq, orbits), dtype=torch.float64, device=DEV)
+        res.append(grads(integ, yt, pt, cot))
+    for gy, gp in res[1:]:
+        assert np.array_equal(gy, res[0][0]) and np.array_equal(gp, res[0][1])

@@ -242,3 +242,92 @@ def test_markstein_division_is_correctly_rounded(ocu, d):
     f.restype = ctypes.c_long
     f.argtypes = [ctypes.c_double, ctypes.c_long, ctypes.c_uint64]
     assert f(d, 2_000_000, 12345) == 0
+
+
+@pytest.mark.parametrize("system,n,npar", [("oculomotor", 6, 6), ("kuramoto", 7, 8)])
+def test_plugin_jacobians_match_central_differences(system, n, npar):
+    """df/dy (calc_jacobian_exact, else calc_jacobian) and df/dp
+    (calc_jacobian_params) of the bundled plugins against central
+    differences of sode_system, at states on both sides of the oculomotor
+    burst curve's switch."""
+    o = Oracle(system)
+    rng = np.random.default_rng(3)
+    for trial in range(6):
+        y = rng.normal(0, 1.0, n)
+        if system == "oculomotor":
+            y[5] = (-1.0) ** trial * rng.uniform(0.2, 2.0)
+            p = np.array(OCU_P) * np.exp(rng.uniform(-0.5, 0.5, 6))
+        else:
+            p = np.concatenate([[rng.uniform(0, 5)], rng.normal(1, 0.1, n)])
+        jac, jp = o.jacobians(y, p)
+        for j in range(n):
+            h = 1e-6 * max(1.0, abs(y[j]))
+            yp, ym = y.copy(), y.copy()
+            yp[j] += h
+            ym[j] -= h
+            fd = (o.rhs(yp, p) - o.rhs(ym, p)) / (2 * h)
+            np.testing.assert_allclose(jac[:, j], fd, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(fd).max()))
+        for k in range(npar):
+            h = 1e-6 * max(1.0, abs(p[k]))
+            pp, pm = p.copy(), p.copy()
+            pp[k] += h
+            pm[k] -= h
+            fd = (o.rhs(y, pp) - o.rhs(y, pm)) / (2 * h)
+            np.testing.assert_allclose(jp[:, k], fd, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(fd).max()))
+
+
+def test_oculomotor_newton_matrix_keeps_the_reference_sign():
+    """calc_jacobian keeps the reference's J[4][5] (examples/oculomotor.cl:75,
+    without the chain rule's minus sign) for the Newton matrix; only the
+    exact Jacobian differs, in that entry only."""
+    o = Oracle("oculomotor")
+    y = np.array([0.1, -0.2, 0.3, 0.4, 0.5, -0.7])
+    jn, _ = o.jacobians(y, OCU_P, exact=False)
+    je, _ = o.jacobians(y, OCU_P, exact=True)
+    diff = np.argwhere(jn != je)
+    assert diff.tolist() == [[4, 5]]
+    assert je[4, 5] == -jn[4, 5]
+
+
+@pytest.mark.parametrize("system,solver", [("oculomotor", "ie"), ("oculomotor", "im"), ("kuramoto", "ie"),
+                                           ("kuramoto", "im")])
+def test_implicit_sensitivity_matches_central_differences(system, solver):
+    """The oracle's forward sensitivity of the implicit solvers (tangent of
+    each step's implicit equation at the converged Newton solution) against
+    central differences of the oracle's own integration, run with a Newton
+    threshold of 1e-14 so that the truncated iteration is smooth to far
+    below the difference step."""
+    o = Oracle(system)
+    rng = np.random.default_rng(5)
+    if system == "oculomotor":
+        orbits, n = 3, 6
+        y0 = np.tile(OCU_Y0, (orbits, 1)) + rng.normal(0, 0.05, (orbits, n))
+        p = np.tile(OCU_P, (orbits, 1)) * np.exp(rng.uniform(-0.3, 0.3, (orbits, 6)))
+        dt, tspan, ks = 1e-4, 0.02, 20
+    else:
+        orbits, n = 3, 5
+        y0 = rng.uniform(0, 2 * np.pi, (orbits, n))
+        p = np.concatenate([rng.uniform(0, 3, (orbits, 1)), rng.normal(1, 0.1, (orbits, n))], 1)
+        dt, tspan, ks = 0.01, 1.0, 10
+    nsnap = o.run(solver, y0, p, dt, tspan, ks)["out"].shape[0]
+    cot = rng.normal(size=(nsnap, n, orbits))
+    gy, gp = o.sensitivity_gradient(solver, y0, p, cot, dt, tspan, ks, newton_tol=1e-14)
+
+    def loss(yv, pv):
+        out = o.run(solver, yv, pv, dt, tspan, ks, newton_tol=1e-14)["out"]
+        return np.einsum("sno,sno->o", out, cot)
+
+    for j in range(n):
+        h = 1e-6
+        yp, ym = y0.copy(), y0.copy()
+        yp[:, j] += h
+        ym[:, j] -= h
+        fd = (loss(yp, p) - loss(ym, p)) / (2 * h)
+        np.testing.assert_allclose(gy[:, j], fd, rtol=1e-6, atol=1e-6 * np.abs(gy).max())
+    for k in range(p.shape[1]):
+        h = 1e-6 * np.abs(p[:, k]).max()
+        pp, pm = p.copy(), p.copy()
+        pp[:, k] += h
+        pm[:, k] -= h
+        fd = (loss(y0, pp) - loss(y0, pm)) / (2 * h)
+        np.testing.assert_allclose(gp[:, k], fd, rtol=1e-6, atol=1e-6 * np.abs(gp).max())
