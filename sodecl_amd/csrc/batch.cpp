@@ -191,8 +191,8 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     if (!bhip(b, hipModuleGetFunction(&b->fn_reset, b->module, "sodecl_batch_reset"), "sodecl_batch_reset"))
         return 0;
     // plain evaluation row + five padded solve rows (one pad per partition) + parameters
-    // evaluation point, parameters, one padded relayout buffer (batch_class.hip)
-    b->lds_bytes = (size_t)(nmax + std::max(b->npar, 1) + (nmax + 64)) * sizeof(double);
+    // evaluation point, four padded rows (residual/solution, band), parameters (batch_class.hip)
+    b->lds_bytes = (size_t)(nmax + 4 * (nmax + 64) + std::max(b->npar, 1)) * sizeof(double);
     int per_cu = 0;
     if (!bhip(b, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, 64, b->lds_bytes),
               "occupancy"))

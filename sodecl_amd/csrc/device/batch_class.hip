@@ -29,11 +29,12 @@
 #define SODECL_TLEN (_numeq_)
 #endif
 
-// Coefficients of row i of the reference's tridiagonal matrix from the
-// plugin's band entries of that row (upi = J[i][i+1], loi = J[i+1][i], both
-// scaled): a_i = upi for 1 <= i < N-2, c_i = loi for i < N-1, else 0.
-SODECL_INLINE double sodecl_tri_a(double upi, int i) { return (i >= 1 && i < _numeq_ - 2) ? upi : 0.0; }
-SODECL_INLINE double sodecl_tri_c(double loi, int i) { return (i < _numeq_ - 1) ? loi : 0.0; }
+// Coefficients of row i of the reference's tridiagonal matrix.
+SODECL_INLINE double sodecl_tri_a(const double* up, int i)
+{
+    return (i >= 1 && i < _numeq_ - 2) ? up[SODECL_TI(i)] : 0.0;
+}
+SODECL_INLINE double sodecl_tri_c(const double* lo, int i) { return (i < _numeq_ - 1) ? lo[SODECL_TI(i)] : 0.0; }
 
 // 1/b from the hardware reciprocal and two Newton-Raphson steps (within an
 // ulp); this solve is not the reference's sweep, so only rounding changes.
@@ -52,17 +53,17 @@ SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up, do
     return inside ? w : identity;
 }
 
-// Partitions that hold rows, and the cyclic-reduction levels they need.
+// Partitions that hold rows, and the cyclic-reduction levels they need
+// (ceil(log2(partitions)); inactive partitions are identity rows).
 #define SODECL_NPART ((_numeq_ + SODECL_BR - 1) / SODECL_BR)
 #define SODECL_PCR_LEVELS (SODECL_NPART > 32 ? 6 : SODECL_NPART > 16 ? 5 : SODECL_NPART > 8 ? 4 : \
                            SODECL_NPART > 4 ? 3 : SODECL_NPART > 2 ? 2 : SODECL_NPART > 1 ? 1 : 0)
 
-// Solves M x = v for the matrix above.  Lane k holds rows r0 = kR .. kR+R-1
-// of the scaled band (lo, di, up) and right-hand side v in registers and
-// receives those rows of x.  All 64 lanes take part.
-SODECL_INLINE void sodecl_tri_solve(const double (&lo)[SODECL_BR], const double (&di)[SODECL_BR],
-                                    const double (&up)[SODECL_BR], const double (&v)[SODECL_BR],
-                                    double (&x)[SODECL_BR], int lane)
+// Solves M x = v for the matrix above; lo, di, up, v, x in LDS (padded
+// layout); x may be v (each lane reads only its own partition's rows of v,
+// all before it writes them); all 64 lanes.
+SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const double* up, const double* v, double* x,
+                                    int lane)
 {
     const int R = SODECL_BR;
     const int r0 = lane * R;
@@ -79,13 +80,13 @@ SODECL_INLINE void sodecl_tri_solve(const double (&lo)[SODECL_BR], const double 
             const int i = r0 + j;
             double al;
             if (j == 0) {
-                al = di[0];
-                be[0] = v[0];
-                ga[0] = sodecl_tri_a(up[0], i);
+                al = di[SODECL_TI(i)];
+                be[0] = v[SODECL_TI(i)];
+                ga[0] = sodecl_tri_a(up, i);
             } else {
-                const double m = sodecl_tri_a(up[j], i) * ia[j - 1];
-                al = di[j] - m * sodecl_tri_c(lo[j - 1], i - 1);
-                be[j] = v[j] - m * be[j - 1];
+                const double m = sodecl_tri_a(up, i) * ia[j - 1];
+                al = di[SODECL_TI(i)] - m * sodecl_tri_c(lo, i - 1);
+                be[j] = v[SODECL_TI(i)] - m * be[j - 1];
                 ga[j] = -m * ga[j - 1];
             }
             ia[j] = sodecl_rcp(al);
@@ -95,7 +96,8 @@ SODECL_INLINE void sodecl_tri_solve(const double (&lo)[SODECL_BR], const double 
 #pragma unroll
     for (int j = SODECL_BR - 2; j >= 0; --j) {
         if (j < nin) {
-            const double ci = sodecl_tri_c(lo[j], r0 + j);
+            const int i = r0 + j;
+            const double ci = sodecl_tri_c(lo, i);
             if (j == nin - 1) {
                 e[j] = be[j] * ia[j];
                 f[j] = -ga[j] * ia[j];
@@ -121,27 +123,16 @@ SODECL_INLINE void sodecl_tri_solve(const double (&lo)[SODECL_BR], const double 
     const double g_nx = __shfl_down(g_first, 1u, SODECL_BG);
     double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
     if (active) {
-        const int jl = nin;   // row r1 within the partition
-        double upl = up[0], lol = lo[0], dil = di[0], vl = v[0];
-#pragma unroll
-        for (int j = 1; j < SODECL_BR; ++j)
-            if (j == jl) {
-                upl = up[j];
-                lol = lo[j];
-                dil = di[j];
-                vl = v[j];
-            }
-        const double ar = sodecl_tri_a(upl, r1);   // zero for row 0
-        const double cr = sodecl_tri_c(lol, r1);
+        const double ar = sodecl_tri_a(up, r1);   // zero for row 0
+        const double cr = sodecl_tri_c(lo, r1);
         const bool has_next = r1 + 1 < _numeq_;
         A = ar * f_last;
-        B = dil + ar * g_last + (has_next ? cr * f_nx : 0.0);
+        B = di[SODECL_TI(r1)] + ar * g_last + (has_next ? cr * f_nx : 0.0);
         C = has_next ? cr * g_nx : 0.0;
-        D = vl - ar * e_last - (has_next ? cr * e_nx : 0.0);
+        D = v[SODECL_TI(r1)] - ar * e_last - (has_next ? cr * e_nx : 0.0);
     }
-    // parallel cyclic reduction over the reduced rows (inactive partitions
-    // are identity rows); each lane shares its pivot's reciprocal so one
-    // reciprocal per lane and level suffices
+    // parallel cyclic reduction over the reduced rows; each lane shares
+    // its pivot's reciprocal so one reciprocal per lane and level suffices
 #pragma unroll
     for (int lv = 0; lv < SODECL_PCR_LEVELS; ++lv) {
         const int s = 1 << lv;
@@ -167,49 +158,12 @@ SODECL_INLINE void sodecl_tri_solve(const double (&lo)[SODECL_BR], const double 
     }
     const double X = D * sodecl_rcp(B);
     const double Xm = sodecl_pcr_neighbour(X, lane, 1, true, 0.0);
-#pragma unroll
-    for (int j = 0; j < SODECL_BR; ++j) x[j] = 0.0;
     if (active) {
 #pragma unroll
-        for (int j = 0; j < SODECL_BR; ++j) {
-            if (j < nin) x[j] = e[j] + f[j] * Xm + g[j] * X;
-            else if (j == nin) x[j] = X;
-        }
+        for (int j = 0; j < SODECL_BR - 1; ++j)
+            if (j < nin) x[SODECL_TI(r0 + j)] = e[j] + f[j] * Xm + g[j] * X;
+        x[SODECL_TI(r1)] = X;
     }
-}
-
-// Wavefront-scope ordering of LDS accesses (the block is one wavefront).
-SODECL_INLINE void sodecl_wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Moves one value per component from the strided layout (component
-// lane + 64 c in src[c]) to the partition layout (row lane * R + j in
-// dst[j]) through the LDS buffer buf, or back when to_part is false.  With
-// one row per partition (N <= 64) both layouts are the same and no LDS is
-// used.
-SODECL_INLINE void sodecl_relayout(const double (&src)[SODECL_BR], double (&dst)[SODECL_BR], double* buf, int lane,
-                                   bool to_part)
-{
-#if SODECL_BR == 1
-    dst[0] = src[0];
-#else
-#pragma unroll
-    for (int c = 0; c < SODECL_BR; ++c) {
-        const int comp = to_part ? lane + c * SODECL_BG : lane * SODECL_BR + c;
-        if (comp < _numeq_) buf[SODECL_TI(comp)] = src[c];
-    }
-    sodecl_wave_sync();
-#pragma unroll
-    for (int c = 0; c < SODECL_BR; ++c) {
-        const int comp = to_part ? lane * SODECL_BR + c : lane + c * SODECL_BG;
-        dst[c] = comp < _numeq_ ? buf[SODECL_TI(comp)] : 0.0;
-    }
-    sodecl_wave_sync();   // buf is free again
-#endif
 }
 
 // Integrates one orbit for `steps` steps of implicit Euler (SODECL_SOLVER 3,
@@ -217,15 +171,17 @@ SODECL_INLINE void sodecl_relayout(const double (&src)[SODECL_BR], double (&dst)
 // Newton iteration: stop when the L1 (IE) / L2 (IM) residual norm is below
 // _epsilon1_, at most SODECL_NEWTON_MAXIT updates.  Snapshot j (state after
 // j * _numsteps_ steps) goes to out[j * neq ...].  Returns the Newton
-// updates taken.  LDS: the evaluation point (neq doubles, read by the
-// plugin), the parameters and one relayout buffer (padded, neq + 64).
+// updates taken.
 __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const double* __restrict__ prm,
                                         double* __restrict__ out, int steps, double* lds)
 {
     const int lane = threadIdx.x;
     double* const sh_g = lds;                          // evaluation point (g, or (y+g)/2), plain layout
-    double* const sh_p = lds + _numeq_;
-    double* const sh_t = sh_p + (_numpar_ > 0 ? _numpar_ : 1);   // relayout buffer (padded layout)
+    double* const sh_r = lds + _numeq_;                // residual -> right-hand side -> solution (padded layout)
+    double* const sh_lo = sh_r + SODECL_TLEN;
+    double* const sh_di = sh_lo + SODECL_TLEN;
+    double* const sh_up = sh_di + SODECL_TLEN;
+    double* const sh_p = sh_up + SODECL_TLEN;
     for (int k = lane; k < _numpar_; k += SODECL_BG) sh_p[k] = prm[k];
     double y[SODECL_BR], gg[SODECL_BR];
 #pragma unroll
@@ -245,7 +201,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #pragma unroll
         for (int c = 0; c < SODECL_BR; ++c) gg[c] = y[c];
         for (int it = 0; it < SODECL_NEWTON_MAXIT; ++it) {
-            sodecl_wave_sync();   // previous readers of sh_g are done
+            __syncthreads();   // previous readers of sh_g are done
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
@@ -255,7 +211,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
                 if (comp < _numeq_) sh_g[comp] = (y[c] + gg[c]) * 0.5;
 #endif
             }
-            sodecl_wave_sync();
+            __syncthreads();
             double r[SODECL_BR];
             double err = 0.0;
 #pragma unroll
@@ -278,34 +234,25 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #else
             if (sqrt(err) < _epsilon1_) break;
 #endif
-            // scaled band of this lane's components (strided layout)
-            double los[SODECL_BR], dis[SODECL_BR], ups[SODECL_BR];
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
-                los[c] = 0.0;
-                dis[c] = 1.0;
-                ups[c] = 0.0;
                 if (comp < _numeq_) {
                     double lo, di, up;
                     calc_jacobian_band_i(comp, sh_g, sh_p, &lo, &di, &up);
-                    los[c] = lo * scale * _m_dt_;
-                    ups[c] = up * scale * _m_dt_;
-                    dis[c] = di * scale * _m_dt_ + 1;
+                    sh_lo[SODECL_TI(comp)] = lo * scale * _m_dt_;
+                    sh_up[SODECL_TI(comp)] = up * scale * _m_dt_;
+                    sh_di[SODECL_TI(comp)] = di * scale * _m_dt_ + 1;
+                    sh_r[SODECL_TI(comp)] = r[c];
                 }
             }
-            // to the partition layout of the solve, solve, and back
-            double lop[SODECL_BR], dip[SODECL_BR], upp[SODECL_BR], vp[SODECL_BR], xp[SODECL_BR], xs[SODECL_BR];
-            sodecl_relayout(los, lop, sh_t, lane, true);
-            sodecl_relayout(dis, dip, sh_t, lane, true);
-            sodecl_relayout(ups, upp, sh_t, lane, true);
-            sodecl_relayout(r, vp, sh_t, lane, true);
-            sodecl_tri_solve(lop, dip, upp, vp, xp, lane);
-            sodecl_relayout(xp, xs, sh_t, lane, false);
+            __syncthreads();
+            sodecl_tri_solve(sh_lo, sh_di, sh_up, sh_r, sh_r, lane);
+            __syncthreads();
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
-                if (comp < _numeq_) gg[c] = gg[c] - xs[c];
+                if (comp < _numeq_) gg[c] = gg[c] - sh_r[SODECL_TI(comp)];
             }
             ++updates;
         }

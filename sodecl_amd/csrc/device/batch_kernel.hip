@@ -9,9 +9,8 @@
 // its snapshots [snap][neq_o] at out[ooff[o] ...], steps[o] / ksteps + 1 of
 // them; params[orbits][npar]; newton[o] receives the Newton updates taken.
 
-// At least two wavefronts per SIMD: the register budget of the largest size
-// class (8 rows per lane at 512 equations) is capped at 256 VGPRs, so twice
-// as many of these latency-bound wavefronts are resident as otherwise.
+// At least two wavefronts per SIMD (a register budget of 256 VGPRs) for these
+// latency-bound wavefronts.
 extern "C" __global__ void __launch_bounds__(SODECL_BG) __attribute__((amdgpu_waves_per_eu(2)))
 sodecl_batch(const int* __restrict__ order, const int* __restrict__ cls, const int* __restrict__ steps,
              const long long* __restrict__ yoff, const long long* __restrict__ ooff, const double* __restrict__ y0,
