@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=0, help="orbits in the CPU baseline sample")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched implicit measurement")
+    ap.add_argument("--no-sampling", action="store_true", help="skip the on-device parameter sampling measurement")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run the ahead-of-time streaming copy kernel once over this many MiB "
                          "(reference point for HBM byte counters)")
@@ -165,6 +166,41 @@ def fwd_adjoint_bench(steps):
                 fwd_value=total / t_fwd, fwd_adjoint_value=total / t_all, unit="orbit-steps/s",
                 fwd_ms=t_fwd * 1e3, fwd_adjoint_ms=t_all * 1e3, cost_ratio=t_all / t_fwd,
                 loss=float(loss), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
+
+
+def sampling_bench(reps=5):
+    """On-device draw of the Euler-Maruyama Kuramoto parameter table
+    (262144 orbits x 65 columns: K ~ U(0,5), w_i ~ N(1,0.1), sigma_i ~
+    U(0,0.5)) against numpy's PCG64 draw of the same shape on the host."""
+    import numpy as np
+    import torch
+    from sodecl_amd.sampling import sample
+    orbits, n = 262144, 32
+    dists = [("uniform", 0, 5)] + [("normal", 1.0, 0.1)] * n + [("uniform", 0, 0.5)] * n
+    out = torch.empty((orbits, len(dists)), dtype=torch.float64, device="cuda:0")
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        sample(dists, orbits, 1, out=out)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for r in range(reps):
+            sample(dists, orbits, r + 2, out=out)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    K = rng.uniform(0, 5, (orbits, 1))
+    w = rng.normal(1.0, 0.1, (orbits, n))
+    s = rng.uniform(0, 0.5, (orbits, n))
+    host = np.concatenate([K, w, s], 1)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    values = orbits * len(dists)
+    return dict(workload="kuramoto_em_parameter_table", rows=orbits, cols=len(dists), device_ms=ms,
+                device_values_per_s=values / (ms * 1e-3), host_numpy_ms=host_ms,
+                host_values_per_s=values / (host_ms * 1e-3), bytes_written=values * 8,
+                device_GBps=values * 8 / (ms * 1e-3) / 1e9, finite=bool(np.isfinite(host).all()))
 
 
 def batch_bench(solver, reps=1):
@@ -312,6 +348,11 @@ def main():
         log("batched implicit Brusselator (32768 orbits, mixed sizes and step counts)")
         batch = [batch_bench(3), batch_bench(4)]
 
+    samp = None
+    if not args.no_sampling and args.config == "oculomotor_rk4":
+        log("on-device parameter sampling (262144 x 65)")
+        samp = sampling_bench()
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (reference kernels on host cores)")
@@ -339,7 +380,7 @@ def main():
                 cfg["solver"]],
             "orbits": orbits, "neq": neq, "dt": cfg["dt"], "steps": steps_per_run, "ksteps": cfg["ksteps"],
             "snapshots": nsnap,
-            "schedule": {1: "lane", 2: "group"}.get(sched[0], str(sched[0])), "group_size": sched[1],
+            "schedule": {1: "lane", 2: "group", 4: "team"}.get(sched[0], str(sched[0])), "group_size": sched[1],
             "block_size": sched[2],
         },
         "roofline": {
@@ -356,6 +397,7 @@ def main():
         "plain_kernel": plain,
         "fwd_adjoint": adj,
         "batch_implicit": batch,
+        "param_sampling": samp,
         "speedup_vs_plain": None if plain is None else value / plain["value"],
         "speedup_vs_cpu": None if cpu is None else value / cpu["value"],
         "finite_fraction_final_state": finite,

@@ -190,6 +190,32 @@ int sodecl_adjoint_device(sodecl_ctx* ctx, const double* ckpt_t, const double* c
                           const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
                           void* stream);
 
+/* ---- On-device sampling of parameters and initial states --------------
+ * (no reference counterpart; replaces the host-side draws of the
+ * reference's benchmark scripts, e.g. benchmarks/kuramoto_benchmark.py:49-68)
+ * Fills out[rows][cols] (device memory, row-major: rows = orbits) with
+ * independent draws, column c from dists[c] (host memory).  Entry (r, c)
+ * comes from one Threefry-2x64-20 block with counter {r, c} and key
+ * {seed, 0x5EEDD15C0FFEE5ED}; u0, u1 are its two words converted to
+ * open-interval (0, 1) doubles as the SDE noise does.
+ *   CONSTANT     a
+ *   UNIFORM      a + (b - a) u0                      (one fused multiply-add)
+ *   LOG_UNIFORM  exp(log a + (log b - log a) u0)     (a, b > 0)
+ *   NORMAL       a + b sqrt(-2 log u0) cos(2 pi u1)  (mean a, deviation b)
+ * Enqueued on stream (NULL: the default stream); returns 0 on bad input. */
+enum sodecl_dist_kind {
+    SODECL_DIST_CONSTANT = 0,
+    SODECL_DIST_UNIFORM = 1,
+    SODECL_DIST_LOG_UNIFORM = 2,
+    SODECL_DIST_NORMAL = 3
+};
+typedef struct sodecl_dist {
+    int kind;   /* enum sodecl_dist_kind */
+    double a, b;
+} sodecl_dist;
+int sodecl_sample_device(const sodecl_dist* dists, int rows, int cols, unsigned long long seed, double* out,
+                         void* stream);
+
 /* ---- Batched integration (no reference counterpart) ---------------------
  * Orbits of several system sizes ("size classes", each a value of _numeq_)
  * and individual step counts in one persistent launch, implicit Euler or

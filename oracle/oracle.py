@@ -78,6 +78,8 @@ class Oracle:
         L.orc_sensitivity_gradient.restype = ctypes.c_long
         L.orc_sensitivity_gradient.argtypes = [ctypes.POINTER(_Config), ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                                _dp, _dp]
+        L.orc_sample.argtypes = [ctypes.POINTER(ctypes.c_int), _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                 _dp]
         L.orc_jacobians.restype = ctypes.c_long
         L.orc_jacobians.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
         L.orc_counter_word.restype = ctypes.c_uint64
@@ -138,6 +140,16 @@ class Oracle:
         if self.lib.orc_jacobians(y.size, p.size, _ptr(y), _ptr(p), _ptr(jac), _ptr(jp), int(exact)) < 0:
             raise ValueError("the plugin lacks calc_jacobian / calc_jacobian_params")
         return jac, jp[:, :p.size]
+
+    def sample(self, table, rows, seed):
+        """The on-device sampler restated: table is a list of (kind, a, b)."""
+        kind = np.array([t[0] for t in table], dtype=np.int32)
+        a = np.array([t[1] for t in table], dtype=np.float64)
+        b = np.array([t[2] for t in table], dtype=np.float64)
+        out = np.empty((rows, len(table)))
+        self.lib.orc_sample(kind.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(a), _ptr(b), rows, len(table),
+                            seed, _ptr(out))
+        return out
 
     def threefry(self, ctr, key):
         c = (ctypes.c_uint64 * 2)(*ctr)

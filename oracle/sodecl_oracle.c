@@ -722,3 +722,34 @@ long orc_markstein_mismatches(double d, long n, uint64_t seed)
     }
     return bad;
 }
+
+/* ------------------------------------------------------------------------ */
+/* On-device sampling (sodecl_sample_device): entry (r, c) of out[rows][cols] */
+/* from Threefry-2x64-20 with counter {r, c}, key {seed, 0x5EEDD15C0FFEE5ED}; */
+/* kind 0 constant a, 1 uniform a + (b-a) u0 (one fma), 2 log-uniform        */
+/* exp(log a + (log b - log a) u0), 3 normal a + b sqrt(-2 log u0) cos(2 pi  */
+/* u1).  Restated from the definition the library documents.                */
+/* ------------------------------------------------------------------------ */
+void orc_sample(const int* kind, const double* a, const double* b, int rows, int cols, uint64_t seed, double* out)
+{
+    const uint64_t key[2] = {seed, 0x5EEDD15C0FFEE5EDULL};
+    for (long r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) {
+            double v = a[c];
+            if (kind[c] != 0) {
+                uint64_t ctr[2] = {(uint64_t)r, (uint64_t)c}, w[2];
+                orc_threefry2x64_20(ctr, key, w);
+                double u0 = orc_u01_open_open_64_53(w[0]);
+                if (kind[c] == 1) {
+                    v = fma(b[c] - a[c], u0, a[c]);
+                } else if (kind[c] == 2) {
+                    double la = log(a[c]);
+                    v = exp(fma(log(b[c]) - la, u0, la));
+                } else {
+                    double u1 = orc_u01_open_open_64_53(w[1]);
+                    v = fma(b[c], sqrt(-2.0 * log(u0)) * cos(2.0 * M_PI * u1), a[c]);
+                }
+            }
+            out[r * cols + c] = v;
+        }
+}

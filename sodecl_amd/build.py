@@ -21,7 +21,7 @@ ARCH = os.environ.get("SODECL_AMD_ARCH", "gfx950")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 # Every HIP compile: no packed FP32 instructions.
 NO_PK_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-DEVICE_SOURCES = ["integrators.hip", "adjoint.hip"]
+DEVICE_SOURCES = ["random.hip", "integrators.hip", "adjoint.hip"]
 HOST_SOURCES = ["jit.cpp", "sodecl_capi.cpp", "sodeclmgr.cpp", "adjoint.cpp", "batch.cpp"]
 
 

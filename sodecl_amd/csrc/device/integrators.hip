@@ -1,7 +1,7 @@
 // SODECL MI355X integrators: device code compiled at run time by hipRTC for
 // gfx950, after the macro prelude and the user's system source.
 //
-// Macros provided by the host (sodecl_amd/csrc/source.cpp):
+// Macros provided by the host (sodecl_amd/csrc/jit.cpp):
 //   _numeq_ _numpar_ _numnoi_ _listsize_ _numsteps_ _m_dt_ [_epsilon1_]
 //       the reference's macro contract (src/sodeclmgr.cpp:390-430)
 //   SODECL_SOLVER          0 Euler-Maruyama, 1 Euler, 2 RK4, 3 implicit Euler,
@@ -11,18 +11,22 @@
 //   SODECL_NEWTON_MAXIT    Newton iteration cap (ie.cl:69 uses 1000)
 //   SODECL_HAS_*           optional plugin entry points found in the source
 //
-// Two schedules:
+// Schedules:
 //   sodecl_lane   one orbit per lane; state, parameters and stage vectors
 //                 live in registers (small systems such as the oculomotor)
 //   sodecl_group  SODECL_GROUP lanes per orbit; lane l owns components
 //                 l, l+G, ...; stage vectors are staged in LDS and every lane
 //                 evaluates its components through the plugin's per-component
 //                 entry point (coupled systems such as Kuramoto)
-// Both run many ksteps segments per launch and write the snapshot that
+//   sodecl_team   SODECL_GROUP lanes per orbit, each with the whole state;
+//                 the plugin's team entry point shares the costly terms
+// All run many ksteps segments per launch and write the snapshot that
 // precedes each segment straight into the trajectory buffer, laid out
 // [snapshot][equation][orbit] as the reference's output (sodeclmgr.cpp:979-995).
 
+#ifndef SODECL_INLINE
 #define SODECL_INLINE __device__ __attribute__((always_inline)) inline
+#endif
 
 #if _numeq_ <= 32
 #define SODECL_UNROLL _Pragma("unroll")
@@ -35,7 +39,6 @@
 #define SODECL_IS_SDE (SODECL_SOLVER == 0)
 #define SODECL_IS_IMPLICIT (SODECL_SOLVER == 3 || SODECL_SOLVER == 4)
 
-typedef unsigned long long sodecl_u64;
 
 // a / d for a divisor d whose correctly rounded reciprocal rd = 1/d is known
 // (a constant, or hoisted out of the step loop): q0 = a*rd is within one ulp,
@@ -52,47 +55,6 @@ SODECL_INLINE double sodecl_div(double a, double d, double rd)
     const double r = fma(-q0, d, a);
     const double q1 = fma(r, rd, q0);
     return (__builtin_isfinite(q0) && q0 != 0.0) ? q1 : q0;
-}
-
-// ---------------------------------------------------------------------------
-// Threefry-2x64, 20 rounds (Salmon, Moraes, Dror, Shaw, "Parallel Random
-// Numbers: As Easy as 1, 2, 3", SC'11): Threefish-style add/rotate/xor rounds
-// with the Skein 2x64 rotation schedule and a key injection after every four
-// rounds.  Identical output to Random123's threefry2x64 (threefry.h:178-293),
-// which the reference's SDE caller uses.
-// ---------------------------------------------------------------------------
-SODECL_INLINE sodecl_u64 sodecl_rotl(sodecl_u64 v, int n) { return (v << n) | (v >> (64 - n)); }
-
-#define SODECL_TF_ROUND(R) \
-    x0 += x1;              \
-    x1 = sodecl_rotl(x1, R); \
-    x1 ^= x0;
-
-SODECL_INLINE void sodecl_threefry2x64_20(sodecl_u64 c0, sodecl_u64 c1, sodecl_u64 k0, sodecl_u64 k1,
-                                          sodecl_u64* o0, sodecl_u64* o1)
-{
-    const sodecl_u64 k2 = 0x1BD11BDAA9FC1A22ull ^ k0 ^ k1;
-    sodecl_u64 x0 = c0 + k0;
-    sodecl_u64 x1 = c1 + k1;
-    SODECL_TF_ROUND(16) SODECL_TF_ROUND(42) SODECL_TF_ROUND(12) SODECL_TF_ROUND(31)
-    x0 += k1; x1 += k2 + 1ull;
-    SODECL_TF_ROUND(16) SODECL_TF_ROUND(32) SODECL_TF_ROUND(24) SODECL_TF_ROUND(21)
-    x0 += k2; x1 += k0 + 2ull;
-    SODECL_TF_ROUND(16) SODECL_TF_ROUND(42) SODECL_TF_ROUND(12) SODECL_TF_ROUND(31)
-    x0 += k0; x1 += k1 + 3ull;
-    SODECL_TF_ROUND(16) SODECL_TF_ROUND(32) SODECL_TF_ROUND(24) SODECL_TF_ROUND(21)
-    x0 += k1; x1 += k2 + 4ull;
-    SODECL_TF_ROUND(16) SODECL_TF_ROUND(42) SODECL_TF_ROUND(12) SODECL_TF_ROUND(31)
-    x0 += k2; x1 += k0 + 5ull;
-    *o0 = x0;
-    *o1 = x1;
-}
-
-// 64-bit word -> double in the open interval (0, 1): (1 + (w >> 11)) * (2^53-1) * 2^-106,
-// the open-open 64->53 conversion of Random123's u01.h:129-131.
-SODECL_INLINE double sodecl_u01(sodecl_u64 w)
-{
-    return (double)(1ull + (w >> 11)) * 0x1.fffffffffffffp-54;
 }
 
 // The reference keeps the per-orbit counter as a double and assigns it to the

@@ -337,6 +337,23 @@ PYBIND11_MODULE(sodecl_interface, m)
           py::arg("orbits"), py::arg("equats"), py::arg("nparams"), py::arg("nnoi"), py::arg("dt"), py::arg("tspan"),
           py::arg("ksteps"), py::arg("local_group_size"));
     m.def("num_launches", &sodecl_num_launches);
+    // dists: sequence of (kind, a, b); out: device pointer to rows x len(dists) doubles
+    m.def(
+        "sample_device",
+        [](py::sequence dists, int rows, unsigned long long seed, uintptr_t out, uintptr_t stream) {
+            std::vector<sodecl_dist> d;
+            for (auto item : dists) {
+                auto t = item.cast<py::tuple>();
+                d.push_back(sodecl_dist{t[0].cast<int>(), t[1].cast<double>(), t[2].cast<double>()});
+            }
+            int ok;
+            {
+                py::gil_scoped_release nogil;
+                ok = sodecl_sample_device(d.data(), rows, (int)d.size(), seed, (double*)out, (void*)stream);
+            }
+            if (!ok) throw std::runtime_error("sodecl_sample_device: invalid distribution table or launch failure");
+        },
+        py::arg("dists"), py::arg("rows"), py::arg("seed"), py::arg("out"), py::arg("stream") = 0);
     m.attr("ABI_VERSION") = SODECL_AMD_ABI_VERSION;
     py::class_<Session>(m, "Session")
         .def(py::init<const std::string&, int, int, int, int, int, double, double, int, int, py::dict, bool>(),

@@ -5,6 +5,7 @@
 #include <cstddef>
 
 #include "../../include/sodecl.h"
+#include "device/random.hip"
 
 // Plain streaming copy, 16 bytes per lane per iteration, grid-stride: the
 // reference point for reading HBM traffic counters (bytes fetched and written
@@ -31,4 +32,78 @@ extern "C" int sodecl_copy_device(void* dst, const void* src, size_t bytes, void
     hipLaunchKernelGGL(sodecl_calibration_copy, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        (const double2*)src, (double2*)dst, n);
     return hipGetLastError() == hipSuccess ? 1 : 0;
+}
+
+// Draws of sodecl_sample_device: entry (row, col) of out[rows][cols] comes
+// from one Threefry-2x64-20 block with counter {row, col} and key
+// {seed, SODECL_SAMPLE_KEY1}, so every entry is reproducible on its own
+// whatever the launch shape.  u0, u1 = open-open (0, 1) doubles of the two
+// output words.  The distributions of up to 64 columns travel by value in
+// the kernel arguments (no device allocation; wider tables take several
+// launches).
+#define SODECL_SAMPLE_KEY1 0x5EEDD15C0FFEE5EDull
+#define SODECL_SAMPLE_CHUNK 64
+
+struct sodecl_dist_chunk {
+    sodecl_dist d[SODECL_SAMPLE_CHUNK];
+    int c0, nc;   // first column, columns in this chunk
+};
+
+__global__ void __launch_bounds__(256) sodecl_sample(const sodecl_dist_chunk chunk, int rows, int cols,
+                                                     unsigned long long seed, double* __restrict__ out)
+{
+    const long long n = (long long)rows * chunk.nc;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const int row = (int)(e / chunk.nc);
+        const int k = (int)(e - (long long)row * chunk.nc);
+        const int col = chunk.c0 + k;
+        const sodecl_dist d = chunk.d[k];
+        double v = d.a;
+        if (d.kind != SODECL_DIST_CONSTANT) {
+            sodecl_u64 w0, w1;
+            sodecl_threefry2x64_20((sodecl_u64)row, (sodecl_u64)col, (sodecl_u64)seed, SODECL_SAMPLE_KEY1, &w0, &w1);
+            const double u0 = sodecl_u01(w0);
+            if (d.kind == SODECL_DIST_UNIFORM) {
+                v = fma(d.b - d.a, u0, d.a);
+            } else if (d.kind == SODECL_DIST_LOG_UNIFORM) {
+                const double la = log(d.a);
+                v = exp(fma(log(d.b) - la, u0, la));
+            } else {   // normal: Box-Muller, cosine branch
+                const double u1 = sodecl_u01(w1);
+                v = fma(d.b, sqrt(-2.0 * log(u0)) * cos(2.0 * M_PI * u1), d.a);
+            }
+        }
+        out[(size_t)row * cols + col] = v;
+    }
+}
+
+extern "C" int sodecl_sample_device(const sodecl_dist* dists, int rows, int cols, unsigned long long seed,
+                                    double* out, void* stream)
+{
+    if (!dists || !out || rows < 0 || cols <= 0) return 0;
+    for (int c = 0; c < cols; ++c) {
+        const sodecl_dist& d = dists[c];
+        if (d.kind < SODECL_DIST_CONSTANT || d.kind > SODECL_DIST_NORMAL) return 0;
+        if (d.kind == SODECL_DIST_LOG_UNIFORM && !(d.a > 0 && d.b > 0)) return 0;
+    }
+    if (rows == 0) return 1;
+    int device = 0;
+    (void)hipGetDevice(&device);
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    for (int c0 = 0; c0 < cols; c0 += SODECL_SAMPLE_CHUNK) {
+        sodecl_dist_chunk chunk{};
+        chunk.c0 = c0;
+        chunk.nc = cols - c0 < SODECL_SAMPLE_CHUNK ? cols - c0 : SODECL_SAMPLE_CHUNK;
+        for (int k = 0; k < chunk.nc; ++k) chunk.d[k] = dists[c0 + k];
+        const long long n = (long long)rows * chunk.nc;
+        long long blocks = (n + 255) / 256;
+        const long long cap = (long long)cus * 16;
+        if (blocks > cap) blocks = cap;
+        hipLaunchKernelGGL(sodecl_sample, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, chunk, rows,
+                           cols, seed, out);
+        if (hipGetLastError() != hipSuccess) return 0;
+    }
+    return 1;
 }

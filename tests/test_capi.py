@@ -243,3 +243,13 @@ def test_occupancy_hint_and_lane_block():
     g = si.Session(kur, 1, 1024, 10, 11, 0, 1e-3, 1.0, 10, 0, {"schedule": "group"})
     g.compile_only("gfx950")
     assert "#define SODECL_KERNEL_ATTR __attribute__" in g.program_source()
+
+
+def test_sampler_rejects_bad_tables_before_launch():
+    """Validation happens on the host before any launch (no GPU needed)."""
+    with pytest.raises(RuntimeError):
+        si.sample_device([(2, 0.0, 1.0)], 10, 1, 1 << 20, 0)    # log-uniform needs a, b > 0
+    with pytest.raises(RuntimeError):
+        si.sample_device([(7, 0.0, 1.0)], 10, 1, 1 << 20, 0)    # unknown kind
+    with pytest.raises(RuntimeError):
+        si.sample_device([], 10, 1, 1 << 20, 0)                 # no columns

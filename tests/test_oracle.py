@@ -331,3 +331,44 @@ def test_implicit_sensitivity_matches_central_differences(system, solver):
         pm[:, k] -= h
         fd = (loss(y0, pp) - loss(y0, pm)) / (2 * h)
         np.testing.assert_allclose(gp[:, k], fd, rtol=1e-6, atol=1e-6 * np.abs(gp).max())
+
+
+SAMPLE_KEY1 = 0x5EEDD15C0FFEE5ED
+
+
+def test_sampler_counter_and_key_mapping():
+    """Entry (r, c) of the sampler is Threefry-2x64-20 at counter {r, c},
+    key {seed, 0x5EEDD15C0FFEE5ED}, converted by u01 (both pinned by their
+    known-answer vectors above): uniform(0, 1) returns u0 itself, a constant
+    column returns a, and the normal column follows Box-Muller."""
+    import math
+    o = Oracle("oculomotor")
+    seed = 123456789
+    table = [(1, 0.0, 1.0), (0, 2.5, 0.0), (3, 1.0, 0.1)]
+    out = o.sample(table, 40, seed)
+    for r in (0, 1, 17, 39):
+        w = o.threefry((r, 0), (seed, SAMPLE_KEY1))
+        assert out[r, 0] == o.u01(w[0])
+        assert out[r, 1] == 2.5
+        w = o.threefry((r, 2), (seed, SAMPLE_KEY1))
+        z = math.sqrt(-2.0 * math.log(o.u01(w[0]))) * math.cos(2.0 * math.pi * o.u01(w[1]))
+        assert abs(out[r, 2] - (1.0 + 0.1 * z)) <= 4e-16 * 2
+
+
+def test_sampler_distributions():
+    """Moments and ranges of 200000 draws per column (bounds of 5 standard
+    errors), and independence of columns with equal distributions."""
+    o = Oracle("oculomotor")
+    n = 200000
+    out = o.sample([(1, 2.0, 5.0), (2, 0.5, 2.0), (3, 1.0, 0.1), (3, 1.0, 0.1)], n, 99)
+    u, lu, z1, z2 = out.T
+    assert u.min() > 2.0 and u.max() < 5.0
+    assert abs(u.mean() - 3.5) < 5 * math.sqrt(0.75 / n)
+    assert abs(u.var() - 0.75) < 5 * 0.75 * math.sqrt(0.8 / n)
+    assert lu.min() > 0.5 and lu.max() < 2.0
+    L = np.log(lu)
+    assert abs(L.mean()) < 5 * math.sqrt(math.log(4) ** 2 / 12 / n)
+    for z in (z1, z2):
+        assert abs(z.mean() - 1.0) < 5 * 0.1 / math.sqrt(n)
+        assert abs(z.std() - 0.1) < 5 * 0.1 / math.sqrt(2 * n)
+    assert abs(np.corrcoef(z1, z2)[0, 1]) < 5 / math.sqrt(n)
