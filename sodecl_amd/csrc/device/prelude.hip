@@ -1,5 +1,21 @@
 // Device helpers visible to the system plugin (emitted before it).
-//
+
+// A double moved across lanes by one DPP control (two 32-bit moves; all
+// source lanes of the patterns used here lie in the same 16-lane row).
+template <int CTRL>
+__device__ inline double sodecl_dpp(double v)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+#define SODECL_DPP_XOR1 0xB1          // quad_perm [1,0,3,2]
+#define SODECL_DPP_XOR2 0x4E          // quad_perm [2,3,0,1]
+#define SODECL_DPP_XOR3 0x1B          // quad_perm [3,2,1,0]
+#define SODECL_DPP_HALF_MIRROR 0x141  // lane i <- 7 - i within each 8 lanes
+#define SODECL_DPP_MIRROR 0x140       // lane i <- 15 - i within each 16 lanes
+
 // Under the group schedule SODECL_GROUP consecutive lanes of a wavefront
 // share one orbit.  A plugin entry point that the whole group calls together
 // (sode_system_vjp_group) may use:
@@ -11,12 +27,21 @@
 #if SODECL_SCHEDULE == 2
 __device__ inline int sodecl_group_size() { return SODECL_GROUP; }
 __device__ inline int sodecl_lane() { return (int)(threadIdx.x % SODECL_GROUP); }
-// Butterfly over the group: at every level the two partners add the same two
-// numbers, so every lane ends with bit-identical sums.
+// Reduction over the group in levels of lane distance 1, 2, 4, 8, 16, 32.
+// The first four are DPP moves inside a 16-lane row: xor 1 and xor 2 by quad
+// permutations, then the half-row and row mirrors, which pair every lane with
+// the opposite block of the previous level (whose lanes all hold the same
+// partial sum) exactly as xor 4 and xor 8 would; the last two are cross-lane
+// permutes.  At every level the two partners add the same two numbers, so
+// every lane ends with a bit-identical sum.
 __device__ inline double sodecl_group_sum(double v)
 {
-#pragma unroll
-    for (int o = SODECL_GROUP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, SODECL_GROUP);
+    if (SODECL_GROUP >= 2) v += sodecl_dpp<SODECL_DPP_XOR1>(v);
+    if (SODECL_GROUP >= 4) v += sodecl_dpp<SODECL_DPP_XOR2>(v);
+    if (SODECL_GROUP >= 8) v += sodecl_dpp<SODECL_DPP_HALF_MIRROR>(v);
+    if (SODECL_GROUP >= 16) v += sodecl_dpp<SODECL_DPP_MIRROR>(v);
+    if (SODECL_GROUP >= 32) v += __shfl_xor(v, 16, SODECL_GROUP);
+    if (SODECL_GROUP >= 64) v += __shfl_xor(v, 32, SODECL_GROUP);
     return v;
 }
 #define SODECL_GROUP_API 1
@@ -36,22 +61,13 @@ __device__ inline double sodecl_group_sum(double v)
 __device__ inline int sodecl_group_size() { return SODECL_GROUP; }
 __device__ inline int sodecl_lane() { return (int)(threadIdx.x % SODECL_GROUP); }
 // Lane exchange within quads by DPP quad permutations for m = 1, 2, 3
-// (no LDS traffic), cross-lane permute otherwise.  CTRL is the quad_perm
-// selector: lane q of each quad reads lane q ^ m.
-template <int CTRL>
-__device__ inline double sodecl_quad_perm(double v)
-{
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
+// (no LDS traffic), cross-lane permute otherwise.
 __device__ inline double sodecl_group_xor(double v, int m)
 {
     switch (m) {
-    case 1: return sodecl_quad_perm<0xB1>(v);
-    case 2: return sodecl_quad_perm<0x4E>(v);
-    case 3: return sodecl_quad_perm<0x1B>(v);
+    case 1: return sodecl_dpp<SODECL_DPP_XOR1>(v);
+    case 2: return sodecl_dpp<SODECL_DPP_XOR2>(v);
+    case 3: return sodecl_dpp<SODECL_DPP_XOR3>(v);
     default: return __shfl_xor(v, m, SODECL_GROUP);
     }
 }

@@ -42,7 +42,7 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("solver", [1, 2])
-@pytest.mark.parametrize("n,group", [(6, 8), (16, 16), (20, 8), (64, 64)])
+@pytest.mark.parametrize("n,group", [(6, 8), (16, 16), (20, 8), (64, 64), (32, 32), (3, 4), (5, 2)])
 def test_adjoint_vs_forward_sensitivity(solver, n, group):
     orbits = 37
     cfg, integ, y0, p = make(n, orbits, solver, seed=n, group=group)
