@@ -108,7 +108,18 @@ def cpu_baseline(cfg, orbits_sample):
     return dict(value=k * steps / dt, unit="orbit-steps/s", cores=nthreads, kind="reference",
                 sample=f"{k} of {cfg['orbits']} orbits, all {steps} steps with trajectory read-back "
                        f"({dt:.2f} s wall)",
-                seconds=dt)
+                seconds=dt, host=host_cpu_model())
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def fwd_adjoint_bench(steps):
@@ -356,8 +367,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (reference kernels on host cores)")
-        sample = args.cpu_orbits or {"oculomotor_rk4": 8192, "kuramoto_euler": 1024, "kuramoto_em": 64,
-                                     "kuramoto_rk4": 16}[args.config]
+        # about 5-15 s of wall time on 16 host threads
+        sample = args.cpu_orbits or {"oculomotor_rk4": 65536, "kuramoto_euler": 1024, "kuramoto_em": 2048,
+                                     "kuramoto_rk4": 512}[args.config]
         cpu = cpu_baseline(cfg, sample)
 
     result = {

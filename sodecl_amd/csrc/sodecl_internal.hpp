@@ -37,8 +37,8 @@ struct ProgramKey {
     int solver = 2;
     int neq = 1, npar = 0, nnoi = 0, orbits = 1, ksteps = 1;
     double dt = 0.0;
-    int schedule = 1;    // 1 lane, 2 group
-    int group = 64;      // lanes per orbit (group schedule)
+    int schedule = 1;    // 1 lane, 2 group, 4 team
+    int group = 64;      // lanes per orbit (group and team schedules)
     int block = 128;
     int newton_max_iter = 1000;
     double newton_tol = 0.0;     // > 0: _epsilon1_ in place of the reference's 1e-07
