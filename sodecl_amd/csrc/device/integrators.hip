@@ -431,7 +431,7 @@ SODECL_INLINE void sodecl_group_sync()
 // finished from s_i = sum_j pair(i, j) taken in j order, pair(i, j) ==
 // -pair(j, i) exactly), every unordered pair is evaluated once: lane of row
 // i evaluates pair(i, i + d mod N) for d = 1 .. (N-1)/2 (and d = N/2 for the
-// first half of the rows when N is even) plus its diagonal, writes the value
+// first half of the rows when N is even; the diagonal is +0), writes the value
 // to S[i][j] and its negation to S[j][i], and after one barrier sums its own
 // row of S in j order -- the same terms in the same order as the per-row
 // loop, with half the evaluations.  Otherwise each component comes from
@@ -445,7 +445,9 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
     for (int c = 0; c < SODECL_CPL; ++c) {
         const int i = lane + c * SODECL_G;
         if (i < _numeq_) {
-            S[i * SODECL_SSTRIDE + i] = sode_pair(i, i, t, x, pp);
+            // pair(i, i) = -pair(i, i) is zero for finite states (and a
+            // non-finite state makes the whole row non-finite): not evaluated
+            S[i * SODECL_SSTRIDE + i] = 0.0;
             for (int d = 1; d <= half; ++d) {
                 const int j = (i + d) % _numeq_;
                 const double v = sode_pair(i, j, t, x, pp);

@@ -39,7 +39,8 @@ double sode_system_i(int i, double t, double y[_numeq_], double p[_numpar_])
 
 /* Pairwise form: f_i = (sum over j in order of sode_pair(i, j)) * (K/N) + w_i,
  * the same operations as sode_system_i.  sin is odd and y_j - y_i is the
- * exact negation of y_i - y_j, so sode_pair(i, j) == -sode_pair(j, i). */
+ * exact negation of y_i - y_j, so sode_pair(i, j) == -sode_pair(j, i); the
+ * integrator takes sode_pair(i, i) = sin(0) = +0 without calling it. */
 double sode_pair(int i, int j, double t, double y[_numeq_], double p[_numpar_])
 {
     return sin(y[j] - y[i]);
