@@ -63,7 +63,7 @@ sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict
     double* const pp = sh_p[slot];
     double* const gp = sh_gp[slot];
 #if SODECL_USE_PAIRS
-    __shared__ double sh_S[SODECL_OPB][_numeq_ * SODECL_SSTRIDE];
+    __shared__ double sh_S[SODECL_OPB][SODECL_PAIR_LDS];
     double* const S = sh_S[slot];
 #else
     double* const S = nullptr;

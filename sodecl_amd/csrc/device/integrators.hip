@@ -429,36 +429,57 @@ SODECL_INLINE void sodecl_group_sync()
 //
 // With the plugin's pairwise form (sode_pair, sode_system_pairsum: f_i is
 // finished from s_i = sum_j pair(i, j) taken in j order, pair(i, j) ==
-// -pair(j, i) exactly), every unordered pair is evaluated once: lane of row
-// i evaluates pair(i, i + d mod N) for d = 1 .. (N-1)/2 (and d = N/2 for the
-// first half of the rows when N is even; the diagonal is +0), writes the value
-// to S[i][j] and its negation to S[j][i], and after one barrier sums its own
-// row of S in j order -- the same terms in the same order as the per-row
-// loop, with half the evaluations.  Otherwise each component comes from
-// sode_system_i.
+// -pair(j, i) exactly), every unordered pair is evaluated once: the lane of
+// row i evaluates pair(i, i + d mod N) for d = 1 .. (N-1)/2, and d = N/2 for
+// the first half of the rows when N is even; the diagonal is +0.  After one
+// barrier each lane sums its own row in j order -- the same terms in the
+// same order as the per-row loop, with half the evaluations.  Two LDS
+// layouts (SODECL_USE_PAIRS):
+//   1  the full matrix S[N][N+1]: the evaluating lane writes the value to
+//      S[i][j] and its negation to S[j][i]; the row sum reads S[i][j];
+//   2  only the evaluated values, V[d-1][i] = pair(i, i + d mod N), half the
+//      storage: the row sum reads V[(j-i) mod N - 1][i] when that offset is
+//      row i's own, else -V[(i-j) mod N - 1][j].
+// Otherwise each component comes from sode_system_i.
 #if SODECL_USE_PAIRS
+#define SODECL_HALF ((_numeq_ - 1) / 2)
+#define SODECL_EVEN ((_numeq_ % 2) == 0)
+#if SODECL_USE_PAIRS == 1
 #define SODECL_SSTRIDE (_numeq_ + 1)
+#define SODECL_PAIR_LDS (_numeq_ * SODECL_SSTRIDE)
+#else
+#define SODECL_PAIR_LDS (_numeq_ * (SODECL_HALF + (SODECL_EVEN ? 1 : 0)))
+#endif
 SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S, int lane, double (&f)[SODECL_CPL])
 {
-    const int half = (_numeq_ - 1) / 2;
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
         const int i = lane + c * SODECL_G;
         if (i < _numeq_) {
+#if SODECL_USE_PAIRS == 1
             // pair(i, i) = -pair(i, i) is zero for finite states (and a
             // non-finite state makes the whole row non-finite): not evaluated
             S[i * SODECL_SSTRIDE + i] = 0.0;
-            for (int d = 1; d <= half; ++d) {
+#endif
+            for (int d = 1; d <= SODECL_HALF; ++d) {
                 const int j = (i + d) % _numeq_;
                 const double v = sode_pair(i, j, t, x, pp);
+#if SODECL_USE_PAIRS == 1
                 S[i * SODECL_SSTRIDE + j] = v;
                 S[j * SODECL_SSTRIDE + i] = -v;
+#else
+                S[(d - 1) * _numeq_ + i] = v;
+#endif
             }
-            if ((_numeq_ % 2) == 0 && i < _numeq_ / 2) {
+            if (SODECL_EVEN && i < _numeq_ / 2) {
                 const int j = i + _numeq_ / 2;
                 const double v = sode_pair(i, j, t, x, pp);
+#if SODECL_USE_PAIRS == 1
                 S[i * SODECL_SSTRIDE + j] = v;
                 S[j * SODECL_SSTRIDE + i] = -v;
+#else
+                S[(_numeq_ / 2 - 1) * _numeq_ + i] = v;
+#endif
             }
         }
     }
@@ -469,7 +490,21 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
         f[c] = 0.0;
         if (i < _numeq_) {
             double acc = 0;
+#if SODECL_USE_PAIRS == 1
             for (int j = 0; j < _numeq_; ++j) acc = acc + S[i * SODECL_SSTRIDE + j];
+#else
+            // offsets row i owns: 1 .. HALF, and N/2 for i < N/2 when N is even
+            const int own_max = SODECL_HALF + ((SODECL_EVEN && i < _numeq_ / 2) ? 1 : 0);
+            int dd = i == 0 ? 0 : _numeq_ - i;   // (j - i) mod N at j = 0
+            for (int j = 0; j < _numeq_; ++j) {
+                const bool own = dd <= own_max;
+                // dd == 0 (the diagonal) reads a valid slot and discards it
+                const int k = dd == 0 ? i : own ? (dd - 1) * _numeq_ + i : (_numeq_ - dd - 1) * _numeq_ + j;
+                const double v = dd == 0 ? 0.0 : S[k];
+                acc = acc + (own ? v : -v);
+                dd = dd + 1 == _numeq_ ? 0 : dd + 1;
+            }
+#endif
             f[c] = sode_system_pairsum(i, t, x, pp, acc);
         }
     }
@@ -588,7 +623,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
     __shared__ double sh_n[2][SODECL_OPB][SODECL_NNOI_ALLOC];
 #endif
 #if SODECL_USE_PAIRS
-    __shared__ double sh_S[SODECL_OPB][_numeq_ * SODECL_SSTRIDE];
+    __shared__ double sh_S[SODECL_OPB][SODECL_PAIR_LDS];
 #endif
     const int lane = threadIdx.x % SODECL_G;
     const int slot = threadIdx.x / SODECL_G;

@@ -142,7 +142,7 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     o << "#define SODECL_HAS_VJP_GROUP " << (f.vjp_group ? 1 : 0) << "\n";
     o << "#define SODECL_HAS_JACOBIAN_PARAMS " << (f.jacobian_params ? 1 : 0) << "\n";
     o << "#define SODECL_HAS_JACOBIAN_EXACT " << (f.jacobian_exact ? 1 : 0) << "\n";
-    o << "#define SODECL_USE_PAIRS " << (k.schedule == 2 && k.use_pairs && f.pairs ? 1 : 0) << "\n";
+    o << "#define SODECL_USE_PAIRS " << (k.schedule == 2 && f.pairs ? k.use_pairs : 0) << "\n";
     if (k.one_wave_per_simd) o << "#define SODECL_KERNEL_ATTR __attribute__((amdgpu_waves_per_eu(1, 1)))\n";
     o << k.extra_defines;
     // OpenCL C spellings a system source may use

@@ -48,16 +48,30 @@ bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what)
     return false;
 }
 
-// Whether the group schedule evaluates the drift in the plugin's pairwise
-// form.  Automatic choice: only while the per-orbit LDS matrix stays within
-// 16 KiB (N <= 44); larger matrices limit the kernel to about one wavefront
-// per SIMD, which costs more than the halved evaluations save.
-bool pairs_active(const sodecl_ctx* c)
+// LDS bytes per orbit of the pairwise drift's two layouts: the full matrix
+// N x (N+1), or the evaluated values only, N x ceil((N-1)/2 + [N even]).
+long pairs_full_bytes(const sodecl_ctx* c) { return (long)c->neq * (c->neq + 1) * 8; }
+long pairs_half_bytes(const sodecl_ctx* c)
 {
-    if (!c->features.pairs || c->opts.pairwise < 0) return false;
-    if (c->opts.pairwise > 0) return true;
-    return (long)c->neq * (c->neq + 1) * 8 <= 16 * 1024;
+    return (long)c->neq * ((c->neq - 1) / 2 + (c->neq % 2 == 0 ? 1 : 0)) * 8;
 }
+
+// Layout of the group schedule's pairwise drift: 0 none (per-component
+// entry point), 1 full matrix, 2 half storage.  Automatic choice: the full
+// matrix while it is at most 16 KiB per orbit (N <= 44), else half storage
+// while that is at most 20 KiB (N <= 70); larger footprints limit the kernel
+// to about one wavefront per SIMD, which costs more than the halved
+// evaluations save.
+int pairs_layout(const sodecl_ctx* c)
+{
+    if (!c->features.pairs || c->opts.pairwise < 0) return 0;
+    if (c->opts.pairwise == 2) return 2;
+    if (c->opts.pairwise == 3) return 1;
+    if (pairs_full_bytes(c) <= 16 * 1024) return 1;
+    if (c->opts.pairwise == 1 || pairs_half_bytes(c) <= 20 * 1024) return 2;
+    return 0;
+}
+
 
 // Compute units of the context's device (256 on MI355X when it cannot be read).
 int device_cus(const sodecl_ctx* c)
@@ -84,7 +98,7 @@ ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, c
     k.newton_max_iter = ctx->opts.newton_max_iter > 0 ? ctx->opts.newton_max_iter : 1000;
     k.newton_tol = ctx->opts.newton_tol;
     k.fp_contract = ctx->opts.fp_contract;
-    k.use_pairs = (schedule == 2 && pairs_active(ctx)) ? 1 : 0;
+    k.use_pairs = schedule == 2 ? pairs_layout(ctx) : 0;
     // At most one wavefront per SIMD: tell the compiler so.  It may then use
     // the whole register file and schedule for instruction-level
     // parallelism, and the dispatcher cannot stack two of the waves on one
@@ -229,13 +243,14 @@ int default_group(int neq)
     return std::max(g, 4);
 }
 
-// The pairwise drift keeps an N x (N+1) matrix per orbit in LDS.  Largest
-// block (a power-of-two number of orbits) whose matrices fit in 48 KiB; a
-// large bound when the group schedule does not use the pairwise form.
+// The pairwise drift keeps its values per orbit in LDS.  Largest block (a
+// power-of-two number of orbits) whose storage fits in 48 KiB; a large bound
+// when the group schedule does not use the pairwise form.
 int pairs_block_cap(const sodecl_ctx* c, int group)
 {
-    if (!pairs_active(c)) return 1 << 20;
-    const long per_orbit = (long)c->neq * (c->neq + 1) * 8;
+    const int layout = pairs_layout(c);
+    if (!layout) return 1 << 20;
+    const long per_orbit = layout == 1 ? pairs_full_bytes(c) : pairs_half_bytes(c);
     const long opb = std::max(1L, (48L * 1024) / per_orbit);
     long p2 = 1;
     while (p2 * 2 <= opb) p2 *= 2;
@@ -849,6 +864,7 @@ int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
     if (o->group_size < 0 || o->group_size > 64) return fail(c, "group_size must be in 0..64");
     if (o->block_size < 0 || o->block_size > 1024) return fail(c, "block_size must be in 0..1024");
     if (!(o->newton_tol >= 0)) return fail(c, "newton_tol must be non-negative");
+    if (o->pairwise < -1 || o->pairwise > 3) return fail(c, "pairwise must be in -1..3");
     // options that change the program text
     bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract ||
                      o->pairwise != c->opts.pairwise || o->newton_tol != c->opts.newton_tol;

@@ -43,7 +43,7 @@ struct ProgramKey {
     int newton_max_iter = 1000;
     double newton_tol = 0.0;     // > 0: _epsilon1_ in place of the reference's 1e-07
     int fp_contract = 0;
-    int use_pairs = 0;           // group schedule evaluates the drift through the pairwise form
+    int use_pairs = 0;           // pairwise drift under the group schedule: 0 off, 1 full matrix, 2 half storage
     int one_wave_per_simd = 0;   // the launch has at most one wavefront per SIMD (occupancy hint)
     std::string extra_defines;   // additional "#define" lines (adjoint, batch)
 };

@@ -172,24 +172,28 @@ def test_reference_pybind_binding_builds_against_sodeclmgr(tmp_path):
 
 def test_pairwise_form_selected_and_lds_bounded():
     """Kuramoto defines sode_pair/sode_system_pairsum: the group schedule uses
-    the pairwise form when asked to, automatically while the per-orbit
-    N x (N+1) LDS matrix is at most 16 KiB, never when disabled; blocks stay
-    small enough that their matrices fit in 48 KiB."""
+    the pairwise form with the full N x (N+1) LDS matrix while that is at
+    most 16 KiB per orbit, half storage while that is at most 20 KiB, as
+    forced by the option, never when disabled; blocks stay small enough
+    that their storage fits in 48 KiB."""
     path = sodecl_amd.system_path("kuramoto")
-    for n, g in ((64, 64), (32, 32), (10, 16)):
+
+    def layout(n, g, mode):
         s = si.Session(path, 2, 1000, n, n + 1, 0, 1e-3, 1.0, 10, 0,
-                       {"schedule": "group", "group_size": g, "pairwise": 1})
+                       {"schedule": "group", "group_size": g, "pairwise": mode})
         assert s.compile_only("gfx950") > 1000
-        assert "#define SODECL_USE_PAIRS 1" in s.program_source()
+        src = s.program_source()
+        value = int(src.split("#define SODECL_USE_PAIRS ")[1].split()[0])
         sched, group, block = s.launch_config()
-        assert (block // group) * n * (n + 1) * 8 <= 48 * 1024
-        d = si.Session(path, 2, 1000, n, n + 1, 0, 1e-3, 1.0, 10, 0,
-                       {"schedule": "group", "group_size": g, "pairwise": -1})
-        d.compile_only("gfx950")
-        assert "#define SODECL_USE_PAIRS 0" in d.program_source()
-        a = si.Session(path, 2, 1000, n, n + 1, 0, 1e-3, 1.0, 10, 0, {"schedule": "group", "group_size": g})
-        a.compile_only("gfx950")
-        assert ("#define SODECL_USE_PAIRS 1" in a.program_source()) == (n * (n + 1) * 8 <= 16384)
+        per_orbit = {0: 0, 1: n * (n + 1) * 8, 2: n * ((n - 1) // 2 + (n % 2 == 0)) * 8}[value]
+        assert (block // group) * per_orbit <= 48 * 1024 or block == group
+        return value
+
+    for n, g, auto in ((64, 64, 2), (32, 32, 1), (10, 16, 1), (44, 64, 1), (45, 64, 2), (72, 64, 0)):
+        assert layout(n, g, 0) == auto, n
+        assert layout(n, g, -1) == 0
+        assert layout(n, g, 2) == 2
+        assert layout(n, g, 3) == 1
     lane = si.Session(path, 2, 1000, 10, 11, 0, 1e-3, 1.0, 10, 0, {"schedule": "lane"})
     lane.compile_only("gfx950")
     assert "#define SODECL_USE_PAIRS 0" in lane.program_source()

@@ -74,8 +74,9 @@ def test_graph_replay_matches_eager_launches():
 @pytest.mark.parametrize("solver,n,group", [(1, 7, 8), (2, 10, 4), (2, 32, 32), (2, 64, 64), (0, 32, 16), (0, 5, 8)])
 def test_pairwise_drift_bit_identical(solver, n, group):
     """The pairwise form of the Kuramoto drift (each sin(y_j - y_i) evaluated
-    once, its negation mirrored) sums the same terms in the same order as
-    sode_system_i, so trajectories must agree bit for bit."""
+    once, its negation mirrored), in both LDS layouts, sums the same terms
+    in the same order as sode_system_i, so trajectories must agree bit for
+    bit."""
     orbits = 301
     if solver == 0:
         cfg = configs.kuramoto_em(orbits=orbits, n=n, seed=n, tspan=0.2, ksteps=10)
@@ -83,11 +84,12 @@ def test_pairwise_drift_bit_identical(solver, n, group):
         cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=n, dt=0.01, tspan=1.0, ksteps=10)
         cfg["solver"] = solver
     outs = []
-    for mode in (1, -1):
+    for mode in (-1, 3, 2):   # per-component, full matrix, half storage
         s = configs.session_for(cfg, schedule="group", group_size=group, pairwise=mode)
         s.setup(cfg["y0"], cfg["params"], None, cfg.get("counters"))
         outs.append(s.run().copy())
     assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
     if solver != 0:
         ref = Oracle("kuramoto").run({1: "euler", 2: "rk4"}[solver], cfg["y0"], cfg["params"], cfg["dt"],
                                      cfg["tspan"], cfg["ksteps"])["out"]
@@ -101,7 +103,7 @@ def test_pairwise_adjoint_bit_identical(solver):
     cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=4, dt=0.01, tspan=0.5, ksteps=5)
     rng = np.random.default_rng(4)
     res = []
-    for mode in (1, -1):
+    for mode in (2, -1, 3):
         integ = Integrator(cfg["system"], solver, orbits, n, n + 1, cfg["dt"], cfg["tspan"], cfg["ksteps"],
                            schedule="group", group_size=8, pairwise=mode)
         y0 = torch.tensor(cfg["y0"], device=DEV, requires_grad=True)
@@ -111,8 +113,9 @@ def test_pairwise_adjoint_bit_identical(solver):
         (traj * cot).sum().backward()
         torch.cuda.synchronize()
         res.append((traj.detach().cpu().numpy(), y0.grad.cpu().numpy(), p.grad.cpu().numpy(), cot.cpu().numpy()))
-    for a, b in zip(res[0][:3], res[1][:3]):
-        assert np.array_equal(a, b)
+    for r in res[1:]:
+        for a, b in zip(res[0][:3], r[:3]):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("solver", [1, 2])
