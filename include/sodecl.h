@@ -82,9 +82,8 @@ typedef struct sodecl_options {
     size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
     int pairwise;            /* group schedule with a plugin that defines the pairwise drift (sode_pair,
                                 sode_system_pairsum): 0 = automatic (the full N x (N+1) LDS matrix while
-                                it is at most 16 KiB per orbit, else half storage while that is at most
-                                20 KiB), 1 = always (automatic layout), 2 = half storage, 3 = full
-                                matrix, -1 = never */
+                                it is at most 16 KiB per orbit, else off), 1 = always (full matrix, or
+                                half storage above 16 KiB), 2 = half storage, 3 = full matrix, -1 = never */
     double newton_tol;       /* Newton stop threshold of the implicit solvers (_epsilon1_); 0 = 1e-07
                                 (src/sodeclmgr.cpp:424-430) */
 } sodecl_options;

@@ -58,18 +58,18 @@ long pairs_half_bytes(const sodecl_ctx* c)
 
 // Layout of the group schedule's pairwise drift: 0 none (per-component
 // entry point), 1 full matrix, 2 half storage.  Automatic choice: the full
-// matrix while it is at most 16 KiB per orbit (N <= 44), else half storage
-// while that is at most 20 KiB (N <= 70); larger footprints limit the kernel
-// to about one wavefront per SIMD, which costs more than the halved
-// evaluations save.
+// matrix while it is at most 16 KiB per orbit (N <= 44), else none.  Larger
+// matrices limit the kernel to about one wavefront per SIMD, and half
+// storage pays for its footprint with index arithmetic per summed term:
+// measured at N = 64 (RK4, 16384 orbits) neither beats the per-component
+// form, at N = 32 (EM) half storage is 1.3x slower than the full matrix.
 int pairs_layout(const sodecl_ctx* c)
 {
     if (!c->features.pairs || c->opts.pairwise < 0) return 0;
     if (c->opts.pairwise == 2) return 2;
     if (c->opts.pairwise == 3) return 1;
     if (pairs_full_bytes(c) <= 16 * 1024) return 1;
-    if (c->opts.pairwise == 1 || pairs_half_bytes(c) <= 20 * 1024) return 2;
-    return 0;
+    return c->opts.pairwise == 1 ? 2 : 0;
 }
 
 

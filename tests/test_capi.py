@@ -173,9 +173,9 @@ def test_reference_pybind_binding_builds_against_sodeclmgr(tmp_path):
 def test_pairwise_form_selected_and_lds_bounded():
     """Kuramoto defines sode_pair/sode_system_pairsum: the group schedule uses
     the pairwise form with the full N x (N+1) LDS matrix while that is at
-    most 16 KiB per orbit, half storage while that is at most 20 KiB, as
-    forced by the option, never when disabled; blocks stay small enough
-    that their storage fits in 48 KiB."""
+    most 16 KiB per orbit (automatically), half storage when asked for,
+    never when disabled; blocks stay small enough that their storage fits
+    in 48 KiB."""
     path = sodecl_amd.system_path("kuramoto")
 
     def layout(n, g, mode):
@@ -189,8 +189,9 @@ def test_pairwise_form_selected_and_lds_bounded():
         assert (block // group) * per_orbit <= 48 * 1024 or block == group
         return value
 
-    for n, g, auto in ((64, 64, 2), (32, 32, 1), (10, 16, 1), (44, 64, 1), (45, 64, 2), (72, 64, 0)):
+    for n, g, auto in ((64, 64, 0), (32, 32, 1), (10, 16, 1), (44, 64, 1), (45, 64, 0)):
         assert layout(n, g, 0) == auto, n
+        assert layout(n, g, 1) == (1 if auto else 2)
         assert layout(n, g, -1) == 0
         assert layout(n, g, 2) == 2
         assert layout(n, g, 3) == 1
