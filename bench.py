@@ -122,18 +122,20 @@ def host_cpu_model():
     return None
 
 
-def fwd_adjoint_bench(steps):
+def fwd_adjoint_bench(steps, store_stages=False):
     """Kuramoto N=64, RK4, dt=0.01, 2000 steps, 16384 orbits, a snapshot
     every 10 steps; loss = mean sin/cos misfit against the trajectory of
     seeded ground-truth (K, w); gradients w.r.t. K, w and theta0 through the
-    discrete adjoint with ceil(sqrt(2000)) = 45-step checkpoints."""
+    discrete adjoint with ceil(sqrt(2000)) = 45-step checkpoints, or with
+    store_stages every step's stage vectors kept by the forward pass (no
+    recomputation)."""
     import torch
     from sodecl_amd import configs
     from sodecl_amd.torch_ops import Integrator
     cfg = configs.kuramoto_rk4()
     dev = torch.device("cuda:0")
     integ = Integrator(cfg["system"], cfg["solver"], cfg["orbits"], cfg["neq"], cfg["npar"], cfg["dt"],
-                       cfg["tspan"], cfg["ksteps"])
+                       cfg["tspan"], cfg["ksteps"], store_stages=store_stages)
     y0 = torch.tensor(cfg["y0"], device=dev)
     truth = torch.tensor(cfg["params"], device=dev)
     rng = torch.Generator(device="cpu").manual_seed(1)
@@ -172,6 +174,11 @@ def fwd_adjoint_bench(steps):
     torch.cuda.synchronize()
     t_all = (time.perf_counter() - t0) / steps
     total = cfg["orbits"] * integ.session.launches * cfg["ksteps"]
+    if store_stages:
+        return dict(workload="kuramoto_rk4_adjoint_stored", orbits=cfg["orbits"], neq=cfg["neq"],
+                    steps=total // cfg["orbits"], workspace_GB=integ.session.stored_workspace_size() * 8 / 1e9,
+                    fwd_adjoint_value=total / t_all, unit="orbit-steps/s", fwd_adjoint_ms=t_all * 1e3,
+                    loss=float(loss), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
     return dict(workload="kuramoto_rk4_adjoint", orbits=cfg["orbits"], neq=cfg["neq"], steps=total // cfg["orbits"],
                 checkpoints=integ.session.checkpoint_count(integ.ckpt_every), ckpt_every=integ.ckpt_every,
                 fwd_value=total / t_fwd, fwd_adjoint_value=total / t_all, unit="orbit-steps/s",
@@ -353,6 +360,10 @@ def main():
     if not args.no_adjoint and args.config == "oculomotor_rk4":
         log("forward+adjoint (Kuramoto N=64 RK4, 16384 orbits)")
         adj = fwd_adjoint_bench(max(1, min(args.steps, 3)))
+        log("forward+adjoint with stored stage vectors (no recomputation)")
+        stored = fwd_adjoint_bench(max(1, min(args.steps, 3)), store_stages=True)
+        stored["cost_ratio_vs_forward"] = stored["fwd_adjoint_ms"] / adj["fwd_ms"]
+        adj["stored_stages"] = stored
 
     batch = None
     if not args.no_batch and args.config == "oculomotor_rk4":

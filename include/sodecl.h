@@ -190,6 +190,18 @@ int sodecl_run_device_checkpointed(sodecl_ctx* ctx, double* t0, double* y0, cons
 int sodecl_adjoint_device(sodecl_ctx* ctx, const double* ckpt_t, const double* ckpt_y, const double* params,
                           const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
                           void* stream);
+/* Without recomputation (Euler and RK4, group schedule), where memory allows:
+ * sodecl_run_device_stored integrates like sodecl_run_device and also writes
+ * every step's stage vectors to work (sodecl_stored_workspace_size doubles,
+ * orbits x steps x (stages x neq + 1): 67 GB for 16384 orbits of a 64-equation
+ * RK4 over 2000 steps); sodecl_adjoint_device_stored then sweeps backwards
+ * from that workspace.  Gradients are bit-identical to the checkpointed
+ * adjoint's. */
+size_t sodecl_stored_workspace_size(const sodecl_ctx* ctx);
+int sodecl_run_device_stored(sodecl_ctx* ctx, double* t0, double* y0, const double* params, double* traj,
+                             double* work, void* stream);
+int sodecl_adjoint_device_stored(sodecl_ctx* ctx, const double* params, const double* cot, double* grad_y0,
+                                 double* grad_params, double* work, void* stream);
 
 /* ---- On-device sampling of parameters and initial states --------------
  * (no reference counterpart; replaces the host-side draws of the

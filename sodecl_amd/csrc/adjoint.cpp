@@ -84,17 +84,19 @@ int sodecl_run_device_checkpointed(sodecl_ctx* c, double* t0, double* y0, const 
     return run_device_impl(c, t0, y0, params, counters, traj, ckpt_t, ckpt_y, ckpt_every, stream);
 }
 
-int sodecl_adjoint_device(sodecl_ctx* c, const double* ckpt_t, const double* ckpt_y, const double* params,
-                          const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
-                          void* stream)
+}  // extern "C"
+
+namespace {
+
+// Launch of the adjoint kernel; stored: the workspace already holds every
+// step's rows (written by sodecl_run_device_stored), no checkpoints.
+int launch_adjoint(sodecl_ctx* c, const double* ckpt_t, const double* ckpt_y, const double* params,
+                   const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every, int stored,
+                   void* stream)
 {
-    if (!c) return 0;
-    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
-    if (!ckpt_t || !ckpt_y || !cot || !grad_y0 || (c->npar > 0 && (!params || !grad_params)) || !work)
-        return fail(c, "null device pointer");
-    if (ckpt_every <= 0) return fail(c, "ckpt_every must be positive");
     int sched = 0, group = 0, block = 0;
     if (!check_adjoint(c, sched, group, block)) return 0;
+    if (stored && implicit(c)) return fail(c, "stored stage vectors are available for Euler and RK4 only");
     if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
     auto prog = get_program(c, sched, group, block, kAdjointDefines);
     if (!prog) return 0;
@@ -107,12 +109,59 @@ int sodecl_adjoint_device(sodecl_ctx* c, const double* ckpt_t, const double* ckp
     const int opb = implicit(c) ? block : block / group;
     const unsigned grid = (unsigned)((orbits + opb - 1) / opb);
     void* args[] = {(void*)&ckpt_t, (void*)&ckpt_y, (void*)&params, (void*)&cot, &grad_y0, &grad_params,
-                    &work, &orbits, &nsteps, &every};
+                    &work, &orbits, &nsteps, &every, &stored};
     hipStream_t s = (hipStream_t)stream;   // null: HIP's default stream
     if (!hip_ok(c, hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)block, 1, 1, 0, s, args, nullptr),
                 "hipModuleLaunchKernel(adjoint)"))
         return 0;
     return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sodecl_adjoint_device(sodecl_ctx* c, const double* ckpt_t, const double* ckpt_y, const double* params,
+                          const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
+                          void* stream)
+{
+    if (!c) return 0;
+    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
+    if (!ckpt_t || !ckpt_y || !cot || !grad_y0 || (c->npar > 0 && (!params || !grad_params)) || !work)
+        return fail(c, "null device pointer");
+    if (ckpt_every <= 0) return fail(c, "ckpt_every must be positive");
+    return launch_adjoint(c, ckpt_t, ckpt_y, params, cot, grad_y0, grad_params, work, ckpt_every, 0, stream);
+}
+
+size_t sodecl_stored_workspace_size(const sodecl_ctx* c)
+{
+    if (!c || implicit(c)) return 0;
+    const long s = total_steps(c);
+    if (s <= 0 || s > 2147483647L) return 0;
+    return sodecl_adjoint_workspace_size(c, (int)s);
+}
+
+int sodecl_run_device_stored(sodecl_ctx* c, double* t0, double* y0, const double* params, double* traj, double* work,
+                             void* stream)
+{
+    if (!c) return 0;
+    if (!work) return fail(c, "null workspace");
+    int sched = 0, group = 0, block = 0;
+    if (!check_adjoint(c, sched, group, block)) return 0;
+    if (implicit(c)) return fail(c, "stored stage vectors are available for Euler and RK4 only");
+    return run_device_impl(c, t0, y0, params, nullptr, traj, nullptr, nullptr, 0, stream, work);
+}
+
+int sodecl_adjoint_device_stored(sodecl_ctx* c, const double* params, const double* cot, double* grad_y0,
+                                 double* grad_params, double* work, void* stream)
+{
+    if (!c) return 0;
+    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
+    if (!cot || !grad_y0 || (c->npar > 0 && (!params || !grad_params)) || !work)
+        return fail(c, "null device pointer");
+    const long s = total_steps(c);
+    if (s <= 0 || s > 2147483647L) return fail(c, "no steps to differentiate");
+    return launch_adjoint(c, nullptr, nullptr, params, cot, grad_y0, grad_params, work, (int)s, 1, stream);
 }
 
 }  // extern "C"

@@ -10,7 +10,10 @@
 // the forward kernel stores the state every ckpt_every steps (sqrt of the
 // step count by default); here each segment is recomputed from its
 // checkpoint, with the stage vectors of every step written to a per-orbit
-// workspace, and then swept backwards.  The plugin supplies the
+// workspace, and then swept backwards.  Where memory allows (an MI355X holds
+// 288 GB), the forward kernel can instead write every step's stage vectors
+// to a whole-run workspace (stored = 1, one segment): the sweep then needs
+// no recomputation.  The plugin supplies the
 // vector-Jacobian product of its drift, evaluated cooperatively by the group:
 //
 //   void sode_system_vjp_group(double t, double x[], double lam[], double p[],
@@ -39,15 +42,14 @@ SODECL_INLINE void sodecl_group_vjp(double t, const double (&v)[SODECL_CPL], con
     vb ^= 1;
 }
 
-// Per-orbit workspace row layout: for each step of a segment, NSTAGE stage
-// vectors of neq doubles, then the step's t.
-#define SODECL_WROW (SODECL_NSTAGE * _numeq_ + 1)
+// Per-orbit workspace: one row of SODECL_WROW doubles (NSTAGE stage vectors
+// of neq doubles, then the step's t) per step of a segment.
 
 extern "C" __global__ void __launch_bounds__(SODECL_BLOCK)
 sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict__ ckpt_y,
                      const double* __restrict__ params, const double* __restrict__ cot,
                      double* __restrict__ grad_y0, double* __restrict__ grad_p, double* __restrict__ work,
-                     int orbits, int total_steps, int ckpt_every)
+                     int orbits, int total_steps, int ckpt_every, int stored)
 {
     __shared__ double sh_x[2][SODECL_OPB][SODECL_XSTRIDE];
     __shared__ double sh_v[2][SODECL_OPB][SODECL_XSTRIDE];
@@ -90,19 +92,22 @@ sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int sb = seg * ckpt_every;
         const int se = min(total_steps, sb + ckpt_every);
-        double y[SODECL_CPL];
+        // forward recompute of the segment, same arithmetic as the forward
+        // kernel (skipped when the forward kernel stored every step's rows)
+        if (!stored) {
+            double y[SODECL_CPL];
 #pragma unroll
-        for (int c = 0; c < SODECL_CPL; ++c) {
-            const int comp = lane + c * SODECL_G;
-            y[c] = comp < _numeq_ ? ckpt_y[(size_t)seg * slab + (size_t)comp * orbits + ob] : 0.0;
-        }
-        double t = ckpt_t[(size_t)seg * orbits + ob];
-        // forward recompute of the segment, same arithmetic as the forward kernel
-        for (int s = sb; s < se; ++s) {
-            double* row = wk + (size_t)(s - sb) * SODECL_WROW;
-            if (lane == 0) row[SODECL_NSTAGE * _numeq_] = t;
-            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, row);
-            t = t + _m_dt_;
+            for (int c = 0; c < SODECL_CPL; ++c) {
+                const int comp = lane + c * SODECL_G;
+                y[c] = comp < _numeq_ ? ckpt_y[(size_t)seg * slab + (size_t)comp * orbits + ob] : 0.0;
+            }
+            double t = ckpt_t[(size_t)seg * orbits + ob];
+            for (int s = sb; s < se; ++s) {
+                double* row = wk + (size_t)(s - sb) * SODECL_WROW;
+                if (lane == 0) row[SODECL_NSTAGE * _numeq_] = t;
+                sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, row);
+                t = t + _m_dt_;
+            }
         }
         __syncthreads();   // workspace rows written by other lanes are read below
         // backward sweep
@@ -247,8 +252,9 @@ extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
 sodecl_lane_adjoint(const double* __restrict__ ckpt_t, const double* __restrict__ ckpt_y,
                     const double* __restrict__ params, const double* __restrict__ cot,
                     double* __restrict__ grad_y0, double* __restrict__ grad_p, double* __restrict__ work,
-                    int orbits, int total_steps, int ckpt_every)
+                    int orbits, int total_steps, int ckpt_every, int stored)
 {
+    (void)stored;   // always recomputes: stored rows exist for the explicit solvers only
     const int i = blockIdx.x * SODECL_BLOCK + threadIdx.x;
     if (i >= orbits) return;
     double p[SODECL_NPAR_ALLOC], gp[SODECL_NPAR_ALLOC];

@@ -254,10 +254,13 @@ extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
 sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
             double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
             int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
-            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0)
+            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0, double* __restrict__ stage_ws,
+            long long ws_steps)
 {
     const int i = blockIdx.x * SODECL_BLOCK + threadIdx.x;
     if (i >= orbits) return;
+    (void)stage_ws;
+    (void)ws_steps;
     double y[_numeq_], yn[_numeq_], p[SODECL_NPAR_ALLOC];
     SODECL_UNROLL
     for (int m = 0; m < _numeq_; ++m) y[m] = y_state[(size_t)i * _numeq_ + m];
@@ -346,7 +349,8 @@ extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
 sodecl_team(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
             double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
             int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
-            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0)
+            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0, double* __restrict__ stage_ws,
+            long long ws_steps)
 {
     const int gid = blockIdx.x * SODECL_BLOCK + threadIdx.x;
     const int i = gid / SODECL_GROUP;            // orbit: a team exits together
@@ -354,6 +358,8 @@ sodecl_team(double* __restrict__ t_state, double* __restrict__ y_state, const do
     if (i >= orbits) return;
     (void)counters;
     (void)newton;
+    (void)stage_ws;
+    (void)ws_steps;
     double y[_numeq_], yn[_numeq_], p[SODECL_NPAR_ALLOC];
     SODECL_UNROLL
     for (int m = 0; m < _numeq_; ++m) y[m] = y_state[(size_t)i * _numeq_ + m];
@@ -424,6 +430,9 @@ SODECL_INLINE void sodecl_group_sync()
 #define SODECL_OPB (SODECL_BLOCK / SODECL_G)
 #define SODECL_XSTRIDE (_numeq_ | 1)
 #define SODECL_NSTAGE (SODECL_SOLVER == 2 ? 4 : 1)
+// Stage workspace row of one step (adjoint): NSTAGE stage vectors of neq
+// doubles, then the step's t.
+#define SODECL_WROW (SODECL_NSTAGE * _numeq_ + 1)
 
 // Drift of this lane's components at the published stage vector x.
 //
@@ -615,7 +624,8 @@ extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
 sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
              double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
              int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
-             double* __restrict__ ckpt_y, int ckpt_every, long long gstep0)
+             double* __restrict__ ckpt_y, int ckpt_every, long long gstep0, double* __restrict__ stage_ws,
+             long long ws_steps)
 {
     __shared__ double sh_x[2][SODECL_OPB][SODECL_XSTRIDE];
     __shared__ double sh_p[SODECL_OPB][SODECL_NPAR_ALLOC];
@@ -708,7 +718,13 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
             }
             buf ^= 1;
 #else
-            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, nullptr);
+            double* row = nullptr;
+            if (stage_ws && active) {
+                // every step's stage vectors for an adjoint without recomputation
+                row = stage_ws + ((size_t)orbit * (size_t)ws_steps + (size_t)g) * SODECL_WROW;
+                if (lane == 0) row[SODECL_NSTAGE * _numeq_] = t;
+            }
+            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, row);
 #endif
             t = t + _m_dt_;
         }

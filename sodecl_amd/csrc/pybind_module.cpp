@@ -189,6 +189,29 @@ public:
         }
         if (!ok) fail();
     }
+    void run_device_stored(uintptr_t t0, uintptr_t y0, uintptr_t params, uintptr_t traj, uintptr_t work,
+                           uintptr_t stream)
+    {
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_run_device_stored(ctx_, (double*)t0, (double*)y0, (const double*)params, (double*)traj,
+                                          (double*)work, (void*)stream);
+        }
+        if (!ok) fail();
+    }
+    void adjoint_device_stored(uintptr_t params, uintptr_t cot, uintptr_t grad_y0, uintptr_t grad_p, uintptr_t work,
+                               uintptr_t stream)
+    {
+        int ok;
+        {
+            py::gil_scoped_release nogil;
+            ok = sodecl_adjoint_device_stored(ctx_, (const double*)params, (const double*)cot, (double*)grad_y0,
+                                              (double*)grad_p, (double*)work, (void*)stream);
+        }
+        if (!ok) fail();
+    }
+    size_t stored_workspace_size() const { return sodecl_stored_workspace_size(ctx_); }
     void adjoint_device(uintptr_t ckpt_t, uintptr_t ckpt_y, uintptr_t params, uintptr_t cot, uintptr_t grad_y0,
                         uintptr_t grad_p, uintptr_t work, int ckpt_every, uintptr_t stream)
     {
@@ -374,6 +397,11 @@ PYBIND11_MODULE(sodecl_interface, m)
         .def("default_checkpoint_interval", &Session::default_checkpoint_interval)
         .def("checkpoint_count", &Session::checkpoint_count)
         .def("adjoint_workspace_size", &Session::adjoint_workspace_size)
+        .def("run_device_stored", &Session::run_device_stored, py::arg("t0"), py::arg("y0"), py::arg("params"),
+             py::arg("traj"), py::arg("work"), py::arg("stream") = 0)
+        .def("adjoint_device_stored", &Session::adjoint_device_stored, py::arg("params"), py::arg("cot"),
+             py::arg("grad_y0"), py::arg("grad_p"), py::arg("work"), py::arg("stream") = 0)
+        .def("stored_workspace_size", &Session::stored_workspace_size)
         .def("counters", &Session::get_counters)
         .def("newton_updates", &Session::newton_updates)
         .def("launch_config", &Session::launch_config)

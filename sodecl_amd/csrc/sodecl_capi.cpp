@@ -308,7 +308,8 @@ LaunchCfg choose_config(const sodecl_ctx* c)
 // trajectory buffer traj (snapshot rows), final snapshot when final_snap.
 int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const double* p, double* cnt, double* traj,
             long long* newton, int s0, int nseg, int final_snap, hipStream_t stream, double* ckpt_t = nullptr,
-            double* ckpt_y = nullptr, int ckpt_every = 0, long long gstep0 = 0)
+            double* ckpt_y = nullptr, int ckpt_every = 0, long long gstep0 = 0, double* stage_ws = nullptr,
+            long long ws_steps = 0)
 {
     auto prog = get_program(c, cfg.schedule, cfg.group, cfg.block);
     if (!prog) return 0;
@@ -329,7 +330,7 @@ int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const dou
         ckpt_every = 1;
     }
     void* args[] = {&t, &y, (void*)&p, &cnt, &traj, &newton, &orbits, &s0, &nseg, &final_snap,
-                    &ckpt_t, &ckpt_y, &ckpt_every, &gstep0};
+                    &ckpt_t, &ckpt_y, &ckpt_every, &gstep0, &stage_ws, &ws_steps};
     if (!hip_ok(c, hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)cfg.block, 1, 1, 0, stream, args, nullptr),
                 "hipModuleLaunchKernel"))
         return 0;
@@ -676,7 +677,7 @@ int autotune(sodecl_ctx* c)
 namespace sodecl_amd {
 
 int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,
-                    double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream)
+                    double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream, double* stage_ws)
 {
     if (!c) return 0;
     if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
@@ -691,16 +692,19 @@ int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params,
     hipStream_t s = (hipStream_t)stream;
     LaunchCfg cfg = choose_config(c);
     long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
+    if (stage_ws && cfg.schedule != SODECL_SCHEDULE_GROUP)
+        return fail(c, "stored stage vectors need the group schedule");
     if (launches == 0) {
         if (traj) return enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, 0, 0, 1, s);
         return 1;
     }
+    const long long ws_steps = (long long)launches * c->ksteps;
     auto enqueue_all = [&](hipStream_t st) -> int {
         for (long s0 = 0; s0 < launches; s0 += per) {
             int n = (int)std::min(per, launches - s0);
             bool last = s0 + n >= launches;
             if (!enqueue(c, cfg, t0, y0, params, counters, traj, nullptr, (int)s0, n, last ? 1 : 0, st, ckpt_t,
-                         ckpt_y, ckpt_every, (long long)s0 * c->ksteps))
+                         ckpt_y, ckpt_every, (long long)s0 * c->ksteps, stage_ws, ws_steps))
                 return 0;
         }
         return 1;
@@ -711,7 +715,7 @@ int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params,
     // stream; the recording is reused while pointers and shape stay the same.
     std::ostringstream key;
     key << (void*)t0 << (void*)y0 << (const void*)params << (void*)counters << (void*)traj << (void*)ckpt_t
-        << (void*)ckpt_y << ":" << ckpt_every << ":" << launches << ":" << per << ":" << cfg.schedule << ":"
+        << (void*)ckpt_y << (void*)stage_ws << ":" << ckpt_every << ":" << launches << ":" << per << ":" << cfg.schedule << ":"
         << cfg.group << ":" << cfg.block;
     if (c->graph_key != key.str() || !c->graph_exec) {
         if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
@@ -953,7 +957,7 @@ int sodecl_run(sodecl_ctx* c)
 int sodecl_run_device(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,
                       void* stream)
 {
-    return sodecl_amd::run_device_impl(c, t0, y0, params, counters, traj, nullptr, nullptr, 0, stream);
+    return sodecl_amd::run_device_impl(c, t0, y0, params, counters, traj, nullptr, nullptr, 0, stream, nullptr);
 }
 
 size_t sodecl_output_size(const sodecl_ctx* c)

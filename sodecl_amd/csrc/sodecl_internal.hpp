@@ -145,8 +145,10 @@ bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what);
 // Program for the given schedule on the context's device (compiled on first use).
 std::shared_ptr<Program> get_program(sodecl_ctx* ctx, int schedule, int group, int block,
                                      const std::string& extra_defines = std::string());
+// stage_ws: when not null, the group kernel also writes every step's stage
+// vectors there (the adjoint's whole-run workspace; group schedule only).
 int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,
-                    double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream);
+                    double* ckpt_t, double* ckpt_y, int ckpt_every, void* stream, double* stage_ws = nullptr);
 // The schedule, group size and block size runs of this context use.
 int launch_config(const sodecl_ctx* c, int* schedule, int* group, int* block);
 }  // namespace sodecl_amd

@@ -213,3 +213,39 @@ def test_implicit_adjoint_checkpoint_interval_and_default_threshold():
         res.append(grads(integ, yt, pt, cot))
     for gy, gp in res[1:]:
         assert np.array_equal(gy, res[0][0]) and np.array_equal(gp, res[0][1])
+
+
+@pytest.mark.parametrize("solver,n,group", [(1, 16, 16), (2, 64, 64), (2, 20, 8)])
+def test_stored_stage_adjoint_bit_identical_to_checkpointed(solver, n, group):
+    """The adjoint without recomputation (the forward pass stores every
+    step's stage vectors) gives the checkpointed adjoint's gradients and the
+    same trajectory, bit for bit."""
+    orbits = 37
+    cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=n + solver, dt=0.01, tspan=0.5, ksteps=5)
+    rng = np.random.default_rng(n)
+    cot = torch.tensor(rng.normal(size=(11, n, orbits)), device=DEV)
+    res = []
+    for stored in (False, True):
+        integ = Integrator(cfg["system"], solver, orbits, n, n + 1, cfg["dt"], cfg["tspan"], cfg["ksteps"],
+                           store_stages=stored, schedule="group", group_size=group)
+        y0 = torch.tensor(cfg["y0"], device=DEV, requires_grad=True)
+        p = torch.tensor(cfg["params"], device=DEV, requires_grad=True)
+        traj = integ(y0, p)
+        (traj * cot).sum().backward()
+        torch.cuda.synchronize()
+        res.append((traj.detach().cpu().numpy(), y0.grad.cpu().numpy(), p.grad.cpu().numpy()))
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+
+
+def test_stored_stage_adjoint_rejects_a_stale_forward():
+    cfg = configs.kuramoto_rk4(orbits=8, n=6, seed=1, dt=0.01, tspan=0.1, ksteps=5)
+    integ = Integrator(cfg["system"], 2, 8, 6, 7, cfg["dt"], cfg["tspan"], cfg["ksteps"], store_stages=True,
+                       schedule="group")
+    y0 = torch.tensor(cfg["y0"], device=DEV, requires_grad=True)
+    p = torch.tensor(cfg["params"], device=DEV, requires_grad=True)
+    first = integ(y0, p)
+    second = integ(y0, p)
+    second.sum().backward()
+    with pytest.raises(RuntimeError, match="later forward pass"):
+        first.sum().backward()
