@@ -178,12 +178,12 @@ def fwd_adjoint_bench(steps, store_stages=False):
         return dict(workload="kuramoto_rk4_adjoint_stored", orbits=cfg["orbits"], neq=cfg["neq"],
                     steps=total // cfg["orbits"], workspace_GB=integ.session.stored_workspace_size() * 8 / 1e9,
                     fwd_adjoint_value=total / t_all, unit="orbit-steps/s", fwd_adjoint_ms=t_all * 1e3,
-                    loss=float(loss), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
+                    loss=float(loss.detach()), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
     return dict(workload="kuramoto_rk4_adjoint", orbits=cfg["orbits"], neq=cfg["neq"], steps=total // cfg["orbits"],
                 checkpoints=integ.session.checkpoint_count(integ.ckpt_every), ckpt_every=integ.ckpt_every,
                 fwd_value=total / t_fwd, fwd_adjoint_value=total / t_all, unit="orbit-steps/s",
                 fwd_ms=t_fwd * 1e3, fwd_adjoint_ms=t_all * 1e3, cost_ratio=t_all / t_fwd,
-                loss=float(loss), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
+                loss=float(loss.detach()), grad_finite=bool(torch.isfinite(gp).all() and torch.isfinite(gy).all()))
 
 
 def sampling_bench(reps=5):
