@@ -18,6 +18,10 @@
 // of one sweep changes only the rounding.
 
 #define SODECL_BR ((_numeq_ + SODECL_BG - 1) / SODECL_BG)   // rows per partition == components per lane
+// Every lane owns exactly SODECL_BR components / rows (neq a multiple of
+// 64): the bounds tests below are then compile-time true.
+#define SODECL_FULL ((_numeq_ % SODECL_BG) == 0)
+#define SODECL_OWNS(comp) (SODECL_FULL || (comp) < _numeq_)
 // LDS index of row i in the solve's arrays: one pad slot per partition, so
 // the 64 lanes reading row j of their own partitions (stride R + 1 doubles)
 // spread over the LDS banks instead of colliding on a stride of R.
@@ -67,9 +71,9 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
 {
     const int R = SODECL_BR;
     const int r0 = lane * R;
-    const bool active = r0 < _numeq_;
-    const int r1 = active ? min(_numeq_, r0 + R) - 1 : r0;
-    const int nin = active ? r1 - r0 : 0;   // interior rows r0 .. r1-1
+    const bool active = SODECL_FULL || r0 < _numeq_;
+    const int r1 = SODECL_FULL ? r0 + R - 1 : (active ? min(_numeq_, r0 + R) - 1 : r0);
+    const int nin = SODECL_FULL ? R - 1 : (active ? r1 - r0 : 0);   // interior rows r0 .. r1-1
     double ia[SODECL_BR], be[SODECL_BR], ga[SODECL_BR];   // ia = 1/alpha
     double e[SODECL_BR], f[SODECL_BR], g[SODECL_BR];
     // forward elimination through the interior, tracking the coupling to
@@ -187,8 +191,8 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #pragma unroll
     for (int c = 0; c < SODECL_BR; ++c) {
         const int comp = lane + c * SODECL_BG;
-        y[c] = comp < _numeq_ ? y0[comp] : 0.0;
-        if (comp < _numeq_) out[comp] = y[c];
+        y[c] = SODECL_OWNS(comp) ? y0[comp] : 0.0;
+        if (SODECL_OWNS(comp)) out[comp] = y[c];
     }
     double t = 0.0;
     long long updates = 0;
@@ -206,9 +210,9 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
 #if SODECL_SOLVER == 3
-                if (comp < _numeq_) sh_g[comp] = gg[c];
+                if (SODECL_OWNS(comp)) sh_g[comp] = gg[c];
 #else
-                if (comp < _numeq_) sh_g[comp] = (y[c] + gg[c]) * 0.5;
+                if (SODECL_OWNS(comp)) sh_g[comp] = (y[c] + gg[c]) * 0.5;
 #endif
             }
             __syncthreads();
@@ -218,7 +222,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
                 r[c] = 0.0;
-                if (comp < _numeq_) {
+                if (SODECL_OWNS(comp)) {
                     const double fv = sode_system_i(comp, t, sh_g, sh_p);
                     r[c] = gg[c] - y[c] - fv * _m_dt_;
 #if SODECL_SOLVER == 3
@@ -237,7 +241,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
-                if (comp < _numeq_) {
+                if (SODECL_OWNS(comp)) {
                     double lo, di, up;
                     calc_jacobian_band_i(comp, sh_g, sh_p, &lo, &di, &up);
                     sh_lo[SODECL_TI(comp)] = lo * scale * _m_dt_;
@@ -252,7 +256,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
-                if (comp < _numeq_) gg[c] = gg[c] - sh_r[SODECL_TI(comp)];
+                if (SODECL_OWNS(comp)) gg[c] = gg[c] - sh_r[SODECL_TI(comp)];
             }
             ++updates;
         }
@@ -264,7 +268,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_BG;
-                if (comp < _numeq_) o[comp] = y[c];
+                if (SODECL_OWNS(comp)) o[comp] = y[c];
             }
         }
     }
@@ -276,3 +280,5 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #undef SODECL_TLEN
 #undef SODECL_NPART
 #undef SODECL_PCR_LEVELS
+#undef SODECL_FULL
+#undef SODECL_OWNS

@@ -18,24 +18,36 @@
 
 #define BRU_CELLS (_numeq_ / 2)
 
+/* Component i: reaction term of its species plus diffusion along the chain
+ * of the same species (neighbours i - 2 and i + 2, ghost values at the
+ * ends).  Written without a branch on the species, so the lanes of a
+ * wavefront that hold u and v components follow one instruction stream;
+ * only the end cells take the ghost-value branch. */
 double sode_system_i(int i, double t, double y[_numeq_], double p[_numpar_])
 {
     const double A = p[0];
     const double B = p[1];
     const double a = p[2];
     const int k = i / 2;
+    const int odd = i & 1;
     const double u = y[2 * k];
     const double v = y[2 * k + 1];
     const double uuv = u * u * v;
-    if ((i & 1) == 0)
+    const double self = odd ? v : u;
+    double left, right;
+    if (k > 0 && k < BRU_CELLS - 1)
     {
-        const double ul = (k > 0) ? y[2 * k - 2] : A;
-        const double ur = (k < BRU_CELLS - 1) ? y[2 * k + 2] : A;
-        return A - (B + 1) * u + uuv + a * (ul - 2 * u + ur);
+        left = y[i - 2];
+        right = y[i + 2];
     }
-    const double vl = (k > 0) ? y[2 * k - 1] : B / A;
-    const double vr = (k < BRU_CELLS - 1) ? y[2 * k + 3] : B / A;
-    return B * u - uuv + a * (vl - 2 * v + vr);
+    else
+    {
+        const double ghost = odd ? B / A : A;
+        left = (k > 0) ? y[i - 2] : ghost;
+        right = (k < BRU_CELLS - 1) ? y[i + 2] : ghost;
+    }
+    const double react = odd ? B * u - uuv : A - (B + 1) * u + uuv;
+    return react + a * (left - 2 * self + right);
 }
 
 void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
@@ -53,20 +65,16 @@ void calc_jacobian_band_i(int i, double y[_numeq_], double p[_numpar_], double* 
     const double B = p[1];
     const double a = p[2];
     const int k = i / 2;
+    const int odd = i & 1;
     const double u = y[2 * k];
     const double v = y[2 * k + 1];
-    if ((i & 1) == 0)
-    {
-        *diag = -(B + 1) + 2 * u * v - 2 * a;
-        *upper = u * u;          /* J[2k][2k+1]   = d(du_k)/dv_k */
-        *lower = B - 2 * u * v;  /* J[2k+1][2k]   = d(dv_k)/du_k */
-    }
-    else
-    {
-        *diag = -(u * u) - 2 * a;
-        *upper = 0;              /* J[2k+1][2k+2] = d(dv_k)/du_{k+1} */
-        *lower = 0;              /* J[2k+2][2k+1] = d(du_{k+1})/dv_k */
-    }
+    /* even i (u_k):  J[2k][2k] = -(B+1) + 2uv - 2a, J[2k][2k+1] = d(du_k)/dv_k = u^2,
+     *                J[2k+1][2k] = d(dv_k)/du_k = B - 2uv;
+     * odd i (v_k):   J[2k+1][2k+1] = -u^2 - 2a, and no coupling to the next
+     *                cell's u (J[2k+1][2k+2] = J[2k+2][2k+1] = 0) */
+    *diag = odd ? -(u * u) - 2 * a : -(B + 1) + 2 * u * v - 2 * a;
+    *upper = odd ? 0.0 : u * u;
+    *lower = odd ? 0.0 : B - 2 * u * v;
 }
 
 void calc_jacobian_band(double y[_numeq_], double lower[_numeq_], double diag[_numeq_], double upper[_numeq_], double p[_numpar_])
