@@ -554,9 +554,11 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
 }
 
 // Candidate launch configurations for the autotuner: the lane schedule at
-// three block sizes and, when the plugin allows it, the group schedule at
-// every power-of-two group size from the system size's own power of two down
-// to a quarter of it, at two block sizes.
+// three block sizes; when the plugin allows it, the group schedule at every
+// power-of-two group size from the system size's own power of two down to a
+// quarter of it, at two block sizes (capped by the pairwise drift's LDS);
+// and when the plugin has a team entry point, teams of two at three block
+// sizes.
 std::vector<LaunchCfg> tune_candidates(const sodecl_ctx* c)
 {
     std::vector<LaunchCfg> out;
@@ -628,9 +630,17 @@ int autotune(sodecl_ctx* c)
     }
     const size_t n = (size_t)c->orbits;
     double *t = nullptr, *y = nullptr, *cnt = nullptr;
-    if (!hip_ok(c, hipMalloc(&t, n * sizeof(double)), "hipMalloc(tune)")) return 0;
-    if (!hip_ok(c, hipMalloc(&y, n * c->neq * sizeof(double)), "hipMalloc(tune)")) return 0;
-    if (c->d_counters && !hip_ok(c, hipMalloc(&cnt, n * sizeof(double)), "hipMalloc(tune)")) return 0;
+    auto release = [&]() {
+        hipFree(t);
+        hipFree(y);
+        hipFree(cnt);
+    };
+    if (!hip_ok(c, hipMalloc(&t, n * sizeof(double)), "hipMalloc(tune)") ||
+        !hip_ok(c, hipMalloc(&y, n * c->neq * sizeof(double)), "hipMalloc(tune)") ||
+        (c->d_counters && !hip_ok(c, hipMalloc(&cnt, n * sizeof(double)), "hipMalloc(tune)"))) {
+        release();
+        return 0;
+    }
     std::ostringstream rep;
     float best = 1e30f;
     LaunchCfg best_cfg{0, 0, 0};
@@ -659,9 +669,7 @@ int autotune(sodecl_ctx* c)
             best_cfg = cfg;
         }
     }
-    hipFree(t);
-    hipFree(y);
-    hipFree(cnt);
+    release();
     if (!best_cfg.schedule) return fail(c, "autotune: no candidate ran");
     c->tuned_schedule = best_cfg.schedule;
     c->tuned_group = best_cfg.group;
