@@ -685,7 +685,8 @@ long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_co
 /* use for divisions by constants and by orbit parameters: with rd = RN(1/d)*/
 /* q0 = RN(a rd), r = a - q0 d (exact, one fma), q1 = RN(q0 + r rd) equals   */
 /* RN(a / d) (Markstein); zeros, inf, NaN pass as in the kernels.  Counts   */
-/* mismatches over n random normal dividends plus special values.           */
+/* mismatches over n random normal dividends plus special values, and for   */
+/* d > 1 those of the unguarded form over the finite ones.                  */
 /* ------------------------------------------------------------------------ */
 static double orc_markstein(double a, double d, double rd)
 {
@@ -693,6 +694,15 @@ static double orc_markstein(double a, double d, double rd)
     const double r = fma(-q0, d, a);
     const double q1 = fma(r, rd, q0);
     return (isfinite(q0) && q0 != 0.0) ? q1 : q0;
+}
+
+/* The unguarded form for finite dividends (the oculomotor plugin's
+ * ocu_div_finite): residual q0 d - a, so a zero quotient keeps its sign. */
+static double orc_markstein_finite(double a, double d, double rd)
+{
+    const double q0 = a * rd;
+    const double r = fma(q0, d, -a);
+    return fma(-r, rd, q0);
 }
 
 long orc_markstein_mismatches(double d, long n, uint64_t seed)
@@ -704,6 +714,8 @@ long orc_markstein_mismatches(double d, long n, uint64_t seed)
     for (size_t i = 0; i < sizeof special / sizeof special[0]; i++) {
         double q = special[i] / d, m = orc_markstein(special[i], d, rd);
         if (memcmp(&q, &m, 8) != 0) bad++;
+        double f = orc_markstein_finite(special[i], d, rd);
+        if (isfinite(special[i]) && d > 1.0 && memcmp(&q, &f, 8) != 0) bad++;
     }
     double nq = NAN / d, nm = orc_markstein(NAN, d, rd);
     if (!(isnan(nq) && isnan(nm))) bad++;
@@ -719,6 +731,8 @@ long orc_markstein_mismatches(double d, long n, uint64_t seed)
         memcpy(&a, &bits, 8);
         double q = a / d, mm = orc_markstein(a, d, rd);
         if (memcmp(&q, &mm, 8) != 0) bad++;
+        double ff = orc_markstein_finite(a, d, rd);
+        if (d > 1.0 && memcmp(&q, &ff, 8) != 0) bad++;
     }
     return bad;
 }

@@ -69,6 +69,36 @@ double ocu_burst(double x, double a, double b, double at, double bt)
     return pos ? at * (1.00 - e) : -(a / b) * x * e;
 }
 
+/* a / d for finite a and a constant divisor d > 1 given rd = 1/d: the
+ * Markstein step of ocu_div with the residual taken as q0*d - a.  That sign
+ * keeps the sign of a zero quotient, so finite dividends need no
+ * special-value select; an infinite a gives NaN rather than an infinity
+ * (used where the step that follows turns that component into NaN anyway). */
+double ocu_div_finite(double a, double d, double rd)
+{
+    const double q0 = a * rd;
+    const double r = fma(q0, d, -a);
+    return fma(-r, rd, q0);
+}
+
+/* Both burst curves of the right-hand side, B(x) and B(-x), with their
+ * shared work done once.  For x > 0, B(x) is on its saturating side with
+ * exponent (-x)/bt = -|x|/bt and B(-x) on its ramp side with exponent
+ * (-x)/b = -|x|/b; for x < 0 the two swap; for a zero x both are ramps whose
+ * exponent is a zero, whose exp is 1 like exp(-|x|/b).  Every case therefore
+ * needs exactly exp(-|x|/bt) and exp(-|x|/b).  The saturating value
+ * at(1 - exp(-|x|/bt)) serves whichever curve is on that side, and the ramp
+ * values -(a/b) x exp(-|x|/b) of B(x) and -(a/b) (-x) exp(-|x|/b) of B(-x)
+ * are negatives of each other bit for bit (rounding to nearest is symmetric
+ * in sign).  Each result has the bits of ocu_burst's. */
+void ocu_burst_pair(double x, double a, double b, double at, double bt, double* bx, double* bmx)
+{
+    const double sat = at * (1.00 - exp(ocu_div(-fabs(x), bt, 1.0 / bt)));
+    const double ramp = -(a / b) * x * exp(ocu_div(-fabs(x), b, 1.0 / b));
+    *bx = x > 0 ? sat : ramp;
+    *bmx = x < 0 ? sat : -ramp;
+}
+
 /* d(ocu_burst)/dx */
 double ocu_burst_dx(double x, double a, double b, double at, double bt)
 {
@@ -87,7 +117,7 @@ void ocu_rhs(double y[_numeq_], double yout[_numeq_], double p[_numpar_], double
 
     yout[0] = y[1];
     yout[1] = -(rate_sum * y[1]) - (rate_prod * y[0]) + (rate_prod * y[2]) + rate_sum * drive;
-    yout[2] = -ocu_div(y[2], OCU_TN, 1.0 / OCU_TN) + drive;
+    yout[2] = -ocu_div_finite(y[2], OCU_TN, 1.0 / OCU_TN) + drive;
     yout[3] = (1.00 / p[2]) * (-y[3] - (p[3] * y[3] * y[4] * y[4]) + br);
     yout[4] = (1.00 / p[2]) * (-y[4] - (p[3] * y[4] * y[3] * y[3]) + bl);
     yout[5] = -drive;
@@ -95,7 +125,9 @@ void ocu_rhs(double y[_numeq_], double yout[_numeq_], double p[_numpar_], double
 
 void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
 {
-    ocu_rhs(y, yout, p, ocu_burst(y[5], p[0], p[1], p[4], p[5]), ocu_burst(-y[5], p[0], p[1], p[4], p[5]));
+    double br, bl;
+    ocu_burst_pair(y[5], p[0], p[1], p[4], p[5], &br, &bl);
+    ocu_rhs(y, yout, p, br, bl);
 }
 
 #ifdef SODECL_TEAM_API

@@ -226,6 +226,11 @@ def test_plugin_matches_reference_example(solver):
     cfg = configs.oculomotor_rk4(orbits=96, seed=11, dt=1e-4, tspan=0.05, ksteps=25)
     y0 = cfg["y0"].copy()
     y0[:, 5] = np.linspace(-2.0, 2.0, 96)
+    # zero burst inputs of both signs, and zero states whose derivative
+    # terms cancel (the sign of each zero is compared too)
+    y0[0, 5], y0[1, 5] = 0.0, -0.0
+    y0[2:6] = 0.0
+    y0[3, 2] = y0[5, 2] = y0[5, 5] = -0.0
     a = RefKernels("oculomotor", solver).run(y0, cfg["params"], 1e-4, 0.05, 25)
     b = RefKernels("refoculomotor", solver).run(y0, cfg["params"], 1e-4, 0.05, 25)
     np.testing.assert_array_equal(a["out"].view(np.int64), b["out"].view(np.int64))
