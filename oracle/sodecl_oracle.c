@@ -33,6 +33,7 @@
  * as C (oracle/_ref) for every solver including Euler-Maruyama.
  */
 #define _GNU_SOURCE
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -685,8 +686,9 @@ long orc_sensitivity_gradient(const orc_config* c, int orbit_begin, int orbit_co
 /* use for divisions by constants and by orbit parameters: with rd = RN(1/d)*/
 /* q0 = RN(a rd), r = a - q0 d (exact, one fma), q1 = RN(q0 + r rd) equals   */
 /* RN(a / d) (Markstein); zeros, inf, NaN pass as in the kernels.  Counts   */
-/* mismatches over n random normal dividends plus special values, and for   */
-/* d > 1 those of the unguarded form over the finite ones.                  */
+/* mismatches over n random normal dividends plus special values, and those */
+/* of the select-free form against it (subnormal and overflowing quotients, */
+/* infinities and NaN included) within that form's contract.                */
 /* ------------------------------------------------------------------------ */
 static double orc_markstein(double a, double d, double rd)
 {
@@ -696,13 +698,22 @@ static double orc_markstein(double a, double d, double rd)
     return (isfinite(q0) && q0 != 0.0) ? q1 : q0;
 }
 
-/* The unguarded form for finite dividends (the oculomotor plugin's
- * ocu_div_finite): residual q0 d - a, so a zero quotient keeps its sign. */
-static double orc_markstein_finite(double a, double d, double rd)
+/* The select-free form the kernels and the oculomotor plugin use: residual */
+/* q0 d - a clamped below at -DBL_MAX.  Its contract: d >= 1, or a dividend  */
+/* whose sign bit is set.                                                    */
+static double orc_markstein_clamped(double a, double d, double rd)
 {
     const double q0 = a * rd;
-    const double r = fma(q0, d, -a);
+    const double r = fmax(fma(q0, d, -a), -DBL_MAX);
     return fma(-r, rd, q0);
+}
+
+static int orc_clamped_agrees(double a, double d, double rd)
+{
+    if (!(d >= 1.0 || signbit(a))) return 1;
+    double g = orc_markstein(a, d, rd), c = orc_markstein_clamped(a, d, rd);
+    if (isnan(g) || isnan(c)) return isnan(g) && isnan(c);
+    return memcmp(&g, &c, 8) == 0;
 }
 
 long orc_markstein_mismatches(double d, long n, uint64_t seed)
@@ -714,9 +725,12 @@ long orc_markstein_mismatches(double d, long n, uint64_t seed)
     for (size_t i = 0; i < sizeof special / sizeof special[0]; i++) {
         double q = special[i] / d, m = orc_markstein(special[i], d, rd);
         if (memcmp(&q, &m, 8) != 0) bad++;
-        double f = orc_markstein_finite(special[i], d, rd);
-        if (isfinite(special[i]) && d > 1.0 && memcmp(&q, &f, 8) != 0) bad++;
+        if (!orc_clamped_agrees(special[i], d, rd) || !orc_clamped_agrees(-special[i], d, rd)) bad++;
     }
+    const double edge[] = {DBL_MAX, 1e-310, 5e-324, 2.2250738585072014e-308, 1e-300 * d, DBL_MAX * (d < 1 ? d : 1)};
+    for (size_t i = 0; i < sizeof edge / sizeof edge[0]; i++)
+        if (!orc_clamped_agrees(edge[i], d, rd) || !orc_clamped_agrees(-edge[i], d, rd)) bad++;
+    if (!orc_clamped_agrees(NAN, d, rd) || !orc_clamped_agrees(-NAN, d, rd)) bad++;
     double nq = NAN / d, nm = orc_markstein(NAN, d, rd);
     if (!(isnan(nq) && isnan(nm))) bad++;
     for (long i = 0; i < n; i++) {
@@ -731,8 +745,7 @@ long orc_markstein_mismatches(double d, long n, uint64_t seed)
         memcpy(&a, &bits, 8);
         double q = a / d, mm = orc_markstein(a, d, rd);
         if (memcmp(&q, &mm, 8) != 0) bad++;
-        double ff = orc_markstein_finite(a, d, rd);
-        if (d > 1.0 && memcmp(&q, &ff, 8) != 0) bad++;
+        if (!orc_clamped_agrees(a, d, rd) || !orc_clamped_agrees(-a, d, rd)) bad++;
     }
     return bad;
 }

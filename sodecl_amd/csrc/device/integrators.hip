@@ -40,21 +40,22 @@
 #define SODECL_IS_IMPLICIT (SODECL_SOLVER == 3 || SODECL_SOLVER == 4)
 
 
-// a / d for a divisor d whose correctly rounded reciprocal rd = 1/d is known
-// (a constant, or hoisted out of the step loop): q0 = a*rd is within one ulp,
-// the remainder a - q0*d is exact in one fused multiply-add, and one more
-// corrects q0 to the correctly rounded quotient (Markstein's theorem), the
-// same bits as IEEE a / d outside the subnormal range.  The correction is
-// applied to finite nonzero q0 only: zeros keep their sign, infinities and
-// NaN pass through (one class test).  Three full-rate FP64 operations instead
-// of the quarter-rate reciprocal and scale/fixup sequence of a general
-// division.
+// a / d for a divisor d >= 1 whose correctly rounded reciprocal rd = 1/d is
+// known (a constant, or hoisted out of the step loop): q0 = a*rd is within
+// one ulp, the residual q0*d - a is exact in one fused multiply-add, and one
+// more corrects q0 to the correctly rounded quotient (Markstein's theorem),
+// the same bits as IEEE a / d outside the subnormal range.  Special values
+// need no select: with the residual's sign taken this way a zero quotient
+// keeps its sign; an infinite a gives a NaN residual, which the clamp to
+// -DBL_MAX turns into a finite one so that the correction leaves q0 (the
+// infinity) in place; a NaN a stays NaN through q0; d >= 1 rules out an
+// overflowing q0.  Four full-rate FP64 operations instead of the
+// quarter-rate reciprocal and scale/fixup sequence of a general division.
 SODECL_INLINE double sodecl_div(double a, double d, double rd)
 {
     const double q0 = a * rd;
-    const double r = fma(-q0, d, a);
-    const double q1 = fma(r, rd, q0);
-    return (__builtin_isfinite(q0) && q0 != 0.0) ? q1 : q0;
+    const double r = fmax(fma(q0, d, -a), -0x1.fffffffffffffp+1023);
+    return fma(-r, rd, q0);
 }
 
 // The reference keeps the per-orbit counter as a double and assigns it to the

@@ -34,16 +34,20 @@
 #define OCU_TN 25.00
 
 /* a / d given rd = 1/d correctly rounded (d a constant or an orbit
- * parameter, so rd is computed once): q0 = a*rd, exact remainder by one
- * fused multiply-add, one correction (Markstein) -- the bits of IEEE a / d
- * outside the subnormal range, without a general division per call.  Zeros
- * (which keep their sign), infinities and NaN pass uncorrected. */
+ * parameter, so rd is computed once): q0 = a*rd, the exact residual
+ * q0*d - a by one fused multiply-add, one correction (Markstein) -- the bits
+ * of IEEE a / d outside the subnormal range, without a general division per
+ * call.  No select for special values: with the residual's sign taken this
+ * way a zero quotient keeps its sign; an infinite dividend or an overflowing
+ * q0 gives a NaN or -inf residual, which the clamp to -DBL_MAX makes finite so
+ * that the correction leaves q0 in place; NaN stays NaN.  That holds for
+ * d >= 1 or a dividend whose sign bit is set (every call here: the constant
+ * Tn and the burst curve's -|x|). */
 double ocu_div(double a, double d, double rd)
 {
     const double q0 = a * rd;
-    const double r = fma(-q0, d, a);
-    const double q1 = fma(r, rd, q0);
-    return (isfinite(q0) && q0 != 0.0) ? q1 : q0;
+    const double r = fmax(fma(q0, d, -a), -0x1.fffffffffffffp+1023);
+    return fma(-r, rd, q0);
 }
 
 /* Burst-neuron firing curve: saturating exponential for x > 0, a scaled
@@ -67,18 +71,6 @@ double ocu_burst(double x, double a, double b, double at, double bt)
      * modifier; for x = +0 it is -0, whose quotient's exp is 1 as well */
     const double e = exp(ocu_div(-fabs(x), pos ? bt : b, pos ? rbt : rb));
     return pos ? at * (1.00 - e) : -(a / b) * x * e;
-}
-
-/* a / d for finite a and a constant divisor d > 1 given rd = 1/d: the
- * Markstein step of ocu_div with the residual taken as q0*d - a.  That sign
- * keeps the sign of a zero quotient, so finite dividends need no
- * special-value select; an infinite a gives NaN rather than an infinity
- * (used where the step that follows turns that component into NaN anyway). */
-double ocu_div_finite(double a, double d, double rd)
-{
-    const double q0 = a * rd;
-    const double r = fma(q0, d, -a);
-    return fma(-r, rd, q0);
 }
 
 /* Both burst curves of the right-hand side, B(x) and B(-x), with their
@@ -117,7 +109,7 @@ void ocu_rhs(double y[_numeq_], double yout[_numeq_], double p[_numpar_], double
 
     yout[0] = y[1];
     yout[1] = -(rate_sum * y[1]) - (rate_prod * y[0]) + (rate_prod * y[2]) + rate_sum * drive;
-    yout[2] = -ocu_div_finite(y[2], OCU_TN, 1.0 / OCU_TN) + drive;
+    yout[2] = -ocu_div(y[2], OCU_TN, 1.0 / OCU_TN) + drive;
     yout[3] = (1.00 / p[2]) * (-y[3] - (p[3] * y[3] * y[4] * y[4]) + br);
     yout[4] = (1.00 / p[2]) * (-y[4] - (p[3] * y[4] * y[3] * y[3]) + bl);
     yout[5] = -drive;

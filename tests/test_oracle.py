@@ -236,12 +236,15 @@ def test_plugin_matches_reference_example(solver):
     np.testing.assert_array_equal(a["out"].view(np.int64), b["out"].view(np.int64))
 
 
-@pytest.mark.parametrize("d", [6.0, 25.0, 0.15, 9.0, 1.5, 0.0045 * 1.7])
+@pytest.mark.parametrize("d", [6.0, 25.0, 0.15, 9.0, 1.5, 0.0045 * 1.7, -0.3, -7.0, 1e-300])
 def test_markstein_division_is_correctly_rounded(ocu, d):
     """The division the kernels use for constant and per-orbit divisors
     (q0 = a/d via the reciprocal, one exact-remainder correction) gives the
     bits of IEEE a / d: random dividends over the normal range plus zeros,
-    infinities, NaN and exact quotients."""
+    infinities, NaN and exact quotients.  The select-free form (residual
+    clamped at -DBL_MAX) gives the bits of the guarded one wherever d >= 1
+    or the dividend's sign bit is set, subnormal and overflowing quotients,
+    infinities and NaN included."""
     import ctypes
     f = ocu.lib.orc_markstein_mismatches
     f.restype = ctypes.c_long
