@@ -64,7 +64,7 @@ sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict
     const int ob = active ? orbit : 0;   // inactive groups read orbit 0's inputs and store no result
     double* const pp = sh_p[slot];
     double* const gp = sh_gp[slot];
-#if SODECL_USE_PAIRS
+#if SODECL_USE_PAIRS == 1 || SODECL_USE_PAIRS == 2
     __shared__ double sh_S[SODECL_OPB][SODECL_PAIR_LDS];
     double* const S = sh_S[slot];
 #else

@@ -450,10 +450,49 @@ SODECL_INLINE void sodecl_group_sync()
 //   2  only the evaluated values, V[d-1][i] = pair(i, i + d mod N), half the
 //      storage: the row sum reads V[(j-i) mod N - 1][i] when that offset is
 //      row i's own, else -V[(i-j) mod N - 1][j].
+//   3  no storage (one component per lane, G >= N): at offset d the lane of
+//      row i evaluates v = pair(i, i + d mod N), adds v and subtracts the v
+//      of the lane of row i - d mod N, whose pair (i - d, i) is the negation
+//      of row i's term j = i - d; the lane exchange is one bpermute per
+//      32-bit half.  Each row sums the same N - 1 terms, in the order of the
+//      offsets rather than of j, so the drift agrees with the per-row form
+//      to rounding of the sum (not bit for bit).  Without LDS matrices the
+//      kernel keeps the per-component form's occupancy.
 // Otherwise each component comes from sode_system_i.
 #if SODECL_USE_PAIRS
 #define SODECL_HALF ((_numeq_ - 1) / 2)
 #define SODECL_EVEN ((_numeq_ % 2) == 0)
+#if SODECL_USE_PAIRS == 3
+static_assert(SODECL_CPL == 1, "the lane-exchange pairwise drift needs one component per lane");
+#define SODECL_PAIR_LDS 1
+SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S, int lane, double (&f)[SODECL_CPL])
+{
+    (void)S;
+    const int i = lane;
+    const bool row = i < _numeq_;
+    double acc = 0.0;
+    for (int d = 1; d <= SODECL_HALF; ++d) {
+        int j = i + d;
+        if (j >= _numeq_) j -= _numeq_;
+        int src = i - d;
+        if (src < 0) src += _numeq_;
+        const double v = row ? sode_pair(i, j, t, x, pp) : 0.0;
+        // lanes past the last row read some lane of the group and discard it
+        const double w = __shfl(v, src, SODECL_G);
+        acc = acc + v;
+        acc = acc - w;
+    }
+    if (SODECL_EVEN) {
+        // offset N/2: rows below N/2 evaluate, rows from N/2 up take the
+        // negation from row i - N/2
+        const int h = _numeq_ / 2;
+        const double v = (i < h) ? sode_pair(i, i + h, t, x, pp) : 0.0;
+        const double w = __shfl(v, i >= h ? i - h : i, SODECL_G);
+        acc = i < h ? acc + v : acc - w;
+    }
+    f[0] = row ? sode_system_pairsum(i, t, x, pp, acc) : 0.0;
+}
+#else
 #if SODECL_USE_PAIRS == 1
 #define SODECL_SSTRIDE (_numeq_ + 1)
 #define SODECL_PAIR_LDS (_numeq_ * SODECL_SSTRIDE)
@@ -519,6 +558,7 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
         }
     }
 }
+#endif
 #define SODECL_EVAL(T, X, F) sodecl_group_eval(T, X, pp, S, lane, F)
 #else
 SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, int lane, double (&f)[SODECL_CPL])
@@ -633,7 +673,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
 #if SODECL_IS_SDE
     __shared__ double sh_n[2][SODECL_OPB][SODECL_NNOI_ALLOC];
 #endif
-#if SODECL_USE_PAIRS
+#if SODECL_USE_PAIRS == 1 || SODECL_USE_PAIRS == 2
     __shared__ double sh_S[SODECL_OPB][SODECL_PAIR_LDS];
 #endif
     const int lane = threadIdx.x % SODECL_G;
@@ -641,7 +681,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
     const int orbit = blockIdx.x * SODECL_OPB + slot;
     const bool active = orbit < orbits;
     (void)newton;
-#if SODECL_USE_PAIRS
+#if SODECL_USE_PAIRS == 1 || SODECL_USE_PAIRS == 2
     double* const S = sh_S[slot];
 #else
     double* const S = nullptr;

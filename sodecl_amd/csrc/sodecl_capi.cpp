@@ -56,20 +56,29 @@ long pairs_half_bytes(const sodecl_ctx* c)
     return (long)c->neq * ((c->neq - 1) / 2 + (c->neq % 2 == 0 ? 1 : 0)) * 8;
 }
 
-// Layout of the group schedule's pairwise drift: 0 none (per-component
-// entry point), 1 full matrix, 2 half storage.  Automatic choice: the full
-// matrix while it is at most 16 KiB per orbit (N <= 44), else none.  Larger
-// matrices limit the kernel to about one wavefront per SIMD, and half
-// storage pays for its footprint with index arithmetic per summed term:
-// measured at N = 64 (RK4, 16384 orbits) neither beats the per-component
-// form, at N = 32 (EM) half storage is 1.3x slower than the full matrix.
-int pairs_layout(const sodecl_ctx* c)
+// Layout of the group schedule's pairwise drift for group size G: 0 none
+// (per-component entry point), 1 full LDS matrix, 2 half LDS storage, 3 lane
+// exchange without storage (G >= N).  Options: -1 never, 1 always (lane
+// exchange when G >= N, else half storage), 2 half, 3 full, 4 lane exchange
+// (when G >= N, else none).  Automatic choice: the lane exchange when
+// G >= N, else the full matrix while it is at most 16 KiB per orbit, else
+// none.  Measured (MI355X): the lane exchange runs N = 64 RK4 (16384 orbits)
+// 1.77x faster than the per-component form and N = 32 EM 1.11x faster than
+// the full matrix, while LDS matrices above 16 KiB per orbit limit the
+// kernel to about one wavefront per SIMD and lose to the per-component form.
+// Layouts 1 and 2 sum each row in j order (the per-component form's bits),
+// layout 3 in offset order (the same terms, the sum rounded differently).
+int pairs_layout(const sodecl_ctx* c, int group)
 {
     if (!c->features.pairs || c->opts.pairwise < 0) return 0;
-    if (c->opts.pairwise == 2) return 2;
-    if (c->opts.pairwise == 3) return 1;
-    if (pairs_full_bytes(c) <= 16 * 1024) return 1;
-    return c->opts.pairwise == 1 ? 2 : 0;
+    const bool exchange = group >= c->neq;
+    switch (c->opts.pairwise) {
+    case 1: return exchange ? 3 : 2;
+    case 2: return 2;
+    case 3: return 1;
+    case 4: return exchange ? 3 : 0;
+    default: return exchange ? 3 : (pairs_full_bytes(c) <= 16 * 1024 ? 1 : 0);
+    }
 }
 
 
@@ -98,7 +107,7 @@ ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, c
     k.newton_max_iter = ctx->opts.newton_max_iter > 0 ? ctx->opts.newton_max_iter : 1000;
     k.newton_tol = ctx->opts.newton_tol;
     k.fp_contract = ctx->opts.fp_contract;
-    k.use_pairs = schedule == 2 ? pairs_layout(ctx) : 0;
+    k.use_pairs = schedule == 2 ? pairs_layout(ctx, group) : 0;
     // At most one wavefront per SIMD: tell the compiler so.  It may then use
     // the whole register file and schedule for instruction-level
     // parallelism, and the dispatcher cannot stack two of the waves on one
@@ -248,8 +257,8 @@ int default_group(int neq)
 // when the group schedule does not use the pairwise form.
 int pairs_block_cap(const sodecl_ctx* c, int group)
 {
-    const int layout = pairs_layout(c);
-    if (!layout) return 1 << 20;
+    const int layout = pairs_layout(c, group);
+    if (layout != 1 && layout != 2) return 1 << 20;
     const long per_orbit = layout == 1 ? pairs_full_bytes(c) : pairs_half_bytes(c);
     const long opb = std::max(1L, (48L * 1024) / per_orbit);
     long p2 = 1;
@@ -876,7 +885,7 @@ int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
     if (o->group_size < 0 || o->group_size > 64) return fail(c, "group_size must be in 0..64");
     if (o->block_size < 0 || o->block_size > 1024) return fail(c, "block_size must be in 0..1024");
     if (!(o->newton_tol >= 0)) return fail(c, "newton_tol must be non-negative");
-    if (o->pairwise < -1 || o->pairwise > 3) return fail(c, "pairwise must be in -1..3");
+    if (o->pairwise < -1 || o->pairwise > 4) return fail(c, "pairwise must be in -1..4");
     // options that change the program text
     bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract ||
                      o->pairwise != c->opts.pairwise || o->newton_tol != c->opts.newton_tol;

@@ -172,10 +172,12 @@ def test_reference_pybind_binding_builds_against_sodeclmgr(tmp_path):
 
 def test_pairwise_form_selected_and_lds_bounded():
     """Kuramoto defines sode_pair/sode_system_pairsum: the group schedule uses
-    the pairwise form with the full N x (N+1) LDS matrix while that is at
-    most 16 KiB per orbit (automatically), half storage when asked for,
-    never when disabled; blocks stay small enough that their storage fits
-    in 48 KiB."""
+    the pairwise form automatically, by lane exchange (layout 3, no LDS)
+    when the group holds one component per lane, else with the full
+    N x (N+1) LDS matrix while that is at most 16 KiB per orbit; half
+    storage, the full matrix and the lane exchange when asked for, never
+    when disabled; blocks stay small enough that their LDS storage fits in
+    48 KiB."""
     path = sodecl_amd.system_path("kuramoto")
 
     def layout(n, g, mode):
@@ -185,13 +187,17 @@ def test_pairwise_form_selected_and_lds_bounded():
         src = s.program_source()
         value = int(src.split("#define SODECL_USE_PAIRS ")[1].split()[0])
         sched, group, block = s.launch_config()
-        per_orbit = {0: 0, 1: n * (n + 1) * 8, 2: n * ((n - 1) // 2 + (n % 2 == 0)) * 8}[value]
+        assert group == g
+        per_orbit = {0: 0, 1: n * (n + 1) * 8, 2: n * ((n - 1) // 2 + (n % 2 == 0)) * 8, 3: 0}[value]
         assert (block // group) * per_orbit <= 48 * 1024 or block == group
         return value
 
-    for n, g, auto in ((64, 64, 0), (32, 32, 1), (10, 16, 1), (44, 64, 1), (45, 64, 0)):
-        assert layout(n, g, 0) == auto, n
-        assert layout(n, g, 1) == (1 if auto else 2)
+    # (N, G, automatic layout, layout for "always", layout for "exchange")
+    for n, g, auto, always, exch in ((64, 64, 3, 3, 3), (32, 32, 3, 3, 3), (10, 16, 3, 3, 3), (45, 64, 3, 3, 3),
+                                     (64, 16, 0, 2, 0), (32, 16, 1, 2, 0), (44, 32, 1, 2, 0), (45, 32, 0, 2, 0)):
+        assert layout(n, g, 0) == auto, (n, g)
+        assert layout(n, g, 1) == always, (n, g)
+        assert layout(n, g, 4) == exch, (n, g)
         assert layout(n, g, -1) == 0
         assert layout(n, g, 2) == 2
         assert layout(n, g, 3) == 1

@@ -2,7 +2,8 @@
 pairwise drift form.  None may change results: the tuned configuration is
 checked against the oracle at 1e-10, graph replays against the eager
 launches and the pairwise form against the per-component form bit for
-bit."""
+bit (LDS layouts, j-order sums) or at the parity tolerance (lane exchange,
+offset-order sums)."""
 import os
 
 import numpy as np
@@ -94,6 +95,65 @@ def test_pairwise_drift_bit_identical(solver, n, group):
         ref = Oracle("kuramoto").run({1: "euler", 2: "rk4"}[solver], cfg["y0"], cfg["params"], cfg["dt"],
                                      cfg["tspan"], cfg["ksteps"])["out"]
         assert_traj_close(outs[0].reshape(-1, n, orbits), ref, what=f"pairs N={n}")
+
+
+@pytest.mark.parametrize("solver,n,group", [(1, 7, 8), (2, 32, 32), (2, 64, 64), (2, 33, 64), (2, 3, 4),
+                                            (0, 5, 8), (0, 32, 32)])
+def test_pairwise_lane_exchange_within_tolerance(solver, n, group):
+    """The lane-exchange pairwise form (one component per lane, no LDS
+    matrix) sums each row's terms in offset order rather than j order: the
+    drift is the per-component drift up to the rounding of that sum, so
+    trajectories agree with the per-component form and with the oracle at
+    the FP64 parity tolerance, and EM noise counters exactly."""
+    orbits = 301
+    if solver == 0:
+        cfg = configs.kuramoto_em(orbits=orbits, n=n, seed=n, tspan=0.2, ksteps=10)
+        cfg["counters"] = np.random.default_rng(n).integers(0, 2 ** 40, orbits).astype(np.float64)
+    else:
+        cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=n, dt=0.01, tspan=1.0, ksteps=10)
+        cfg["solver"] = solver
+    outs, cnts = [], []
+    for mode in (-1, 4):
+        s = configs.session_for(cfg, schedule="group", group_size=group, pairwise=mode)
+        s.setup(cfg["y0"], cfg["params"], None, cfg.get("counters"))
+        assert "#define SODECL_USE_PAIRS " + ("0" if mode < 0 else "3") in s.program_source()
+        outs.append(s.run().copy().reshape(-1, n, orbits))
+        cnts.append(s.counters().copy() if solver == 0 else None)
+    assert_traj_close(outs[1], outs[0], what=f"exchange vs per-component N={n}")
+    ref = Oracle("kuramoto").run({0: "em", 1: "euler", 2: "rk4"}[solver], cfg["y0"], cfg["params"], cfg["dt"],
+                                 cfg["tspan"], cfg["ksteps"], counters=cfg.get("counters"), nnoi=cfg["nnoi"])
+    assert_traj_close(outs[1], ref["out"], what=f"exchange vs oracle N={n}")
+    if solver == 0:
+        np.testing.assert_array_equal(cnts[1], ref["counters"])
+
+
+def test_pairwise_lane_exchange_adjoint():
+    """Discrete adjoint (RK4, group schedule) with the lane-exchange drift:
+    its recomputation repeats the forward pass bit for bit, so gradients
+    with stored stage vectors equal the checkpointed ones, and both agree
+    with the per-component form's gradients at the FP64 parity tolerance."""
+    from sodecl_amd.torch_ops import Integrator
+    n, orbits = 32, 70
+    cfg = configs.kuramoto_rk4(orbits=orbits, n=n, seed=5, dt=0.01, tspan=0.5, ksteps=5)
+    cot = None
+    res = []
+    for mode, stored in ((-1, False), (4, False), (4, True)):
+        integ = Integrator(cfg["system"], 2, orbits, n, n + 1, cfg["dt"], cfg["tspan"], cfg["ksteps"],
+                           schedule="group", group_size=32, pairwise=mode, store_stages=stored)
+        y0 = torch.tensor(cfg["y0"], device=DEV, requires_grad=True)
+        p = torch.tensor(cfg["params"], device=DEV, requires_grad=True)
+        if cot is None:
+            cot = torch.tensor(np.random.default_rng(5).normal(size=(integ.nsnap, n, orbits)), device=DEV)
+        traj = integ(y0, p)
+        (traj * cot).sum().backward()
+        torch.cuda.synchronize()
+        res.append((traj.detach().cpu().numpy(), y0.grad.cpu().numpy(), p.grad.cpu().numpy()))
+    for a, b in zip(res[1], res[2]):
+        assert np.array_equal(a, b)
+    assert_traj_close(res[1][0], res[0][0], what="trajectory")
+    for a, b, what in ((res[1][1], res[0][1], "grad y0"), (res[1][2], res[0][2], "grad params")):
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        assert err <= 1e-10, f"{what}: {err:.2e}"
 
 
 @pytest.mark.parametrize("solver", [1, 2])
