@@ -206,6 +206,25 @@ def test_pairwise_form_selected_and_lds_bounded():
     assert "#define SODECL_USE_PAIRS 0" in lane.program_source()
 
 
+def test_group_blocks_fill_the_chip():
+    """Group-schedule blocks shrink (down to one wavefront) while a launch
+    would give fewer blocks than the 256 compute units; an explicit block
+    size is kept."""
+    path = sodecl_amd.system_path("kuramoto")
+
+    def cfg(orbits, n, **opts):
+        s = si.Session(path, 1, orbits, n, n + 1, 0, 1e-2, 1.0, 10, 0, dict(schedule="group", **opts))
+        s.compile_only("gfx950")
+        return s.launch_config()
+
+    assert cfg(1024, 10) == (2, 16, 64)
+    assert cfg(2048, 10) == (2, 16, 128)
+    assert cfg(4096, 10) == (2, 16, 256)
+    assert cfg(16384, 64) == (2, 64, 256)
+    assert cfg(1024, 10, block_size=256) == (2, 16, 256)
+    assert cfg(16, 64) == (2, 64, 64)
+
+
 def test_team_schedule_selection():
     """The team schedule needs the plugin's sode_system_team (oculomotor has
     it, Kuramoto does not) and an explicit non-stochastic solver.  It is
