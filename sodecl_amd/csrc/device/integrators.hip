@@ -439,12 +439,13 @@ SODECL_INLINE void sodecl_group_sync()
 //
 // With the plugin's pairwise form (sode_pair, sode_system_pairsum: f_i is
 // finished from s_i = sum_j pair(i, j) taken in j order, pair(i, j) ==
-// -pair(j, i) exactly), every unordered pair is evaluated once: the lane of
-// row i evaluates pair(i, i + d mod N) for d = 1 .. (N-1)/2, and d = N/2 for
-// the first half of the rows when N is even; the diagonal is +0.  After one
-// barrier each lane sums its own row in j order -- the same terms in the
-// same order as the per-row loop, with half the evaluations.  Two LDS
-// layouts (SODECL_USE_PAIRS):
+// -pair(j, i) exactly), every unordered pair is evaluated once: the pairs
+// are (i, i + d mod N) for d = 1 .. (N-1)/2, and d = N/2 for the first half
+// of the rows when N is even; the diagonal is +0.  In the two LDS layouts
+// the pairs are spread over all G lanes of the group (G may exceed N, so a
+// small system can use a whole wavefront per orbit); after one barrier each
+// lane sums its own row in j order -- the same terms in the same order as
+// the per-row loop, with half the evaluations.  Layouts (SODECL_USE_PAIRS):
 //   1  the full matrix S[N][N+1]: the evaluating lane writes the value to
 //      S[i][j] and its negation to S[j][i]; the row sum reads S[i][j];
 //   2  only the evaluated values, V[d-1][i] = pair(i, i + d mod N), half the
@@ -499,38 +500,35 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
 #else
 #define SODECL_PAIR_LDS (_numeq_ * (SODECL_HALF + (SODECL_EVEN ? 1 : 0)))
 #endif
+// Unordered pairs in a flattened order, k = (d - 1) N + i for the pair
+// (i, i + d mod N): offsets d = 1 .. HALF for every row, then d = N/2 for
+// rows i < N/2 when N is even.
+#define SODECL_NPAIRS (_numeq_ * SODECL_HALF + (SODECL_EVEN ? _numeq_ / 2 : 0))
 SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S, int lane, double (&f)[SODECL_CPL])
 {
+#if SODECL_USE_PAIRS == 1
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c) {
         const int i = lane + c * SODECL_G;
-        if (i < _numeq_) {
-#if SODECL_USE_PAIRS == 1
-            // pair(i, i) = -pair(i, i) is zero for finite states (and a
-            // non-finite state makes the whole row non-finite): not evaluated
-            S[i * SODECL_SSTRIDE + i] = 0.0;
+        // pair(i, i) = -pair(i, i) is zero for finite states (and a
+        // non-finite state makes the whole row non-finite): not evaluated
+        if (i < _numeq_) S[i * SODECL_SSTRIDE + i] = 0.0;
+    }
 #endif
-            for (int d = 1; d <= SODECL_HALF; ++d) {
-                const int j = (i + d) % _numeq_;
-                const double v = sode_pair(i, j, t, x, pp);
+    // the pairs go round-robin to all G lanes of the group (lanes past the
+    // last row evaluate pairs too), at most ceil(NPAIRS / G) each
+    for (int k = lane; k < SODECL_NPAIRS; k += SODECL_G) {
+        const int d = k / _numeq_ + 1;
+        const int i = k - (d - 1) * _numeq_;
+        int j = i + d;
+        if (j >= _numeq_) j -= _numeq_;
+        const double v = sode_pair(i, j, t, x, pp);
 #if SODECL_USE_PAIRS == 1
-                S[i * SODECL_SSTRIDE + j] = v;
-                S[j * SODECL_SSTRIDE + i] = -v;
+        S[i * SODECL_SSTRIDE + j] = v;
+        S[j * SODECL_SSTRIDE + i] = -v;
 #else
-                S[(d - 1) * _numeq_ + i] = v;
+        S[k] = v;
 #endif
-            }
-            if (SODECL_EVEN && i < _numeq_ / 2) {
-                const int j = i + _numeq_ / 2;
-                const double v = sode_pair(i, j, t, x, pp);
-#if SODECL_USE_PAIRS == 1
-                S[i * SODECL_SSTRIDE + j] = v;
-                S[j * SODECL_SSTRIDE + i] = -v;
-#else
-                S[(_numeq_ / 2 - 1) * _numeq_ + i] = v;
-#endif
-            }
-        }
     }
     sodecl_group_sync();
 #pragma unroll

@@ -58,26 +58,31 @@ long pairs_half_bytes(const sodecl_ctx* c)
 
 // Layout of the group schedule's pairwise drift for group size G: 0 none
 // (per-component entry point), 1 full LDS matrix, 2 half LDS storage, 3 lane
-// exchange without storage (G >= N).  Options: -1 never, 1 always (lane
-// exchange when G >= N, else half storage), 2 half, 3 full, 4 lane exchange
-// (when G >= N, else none).  Automatic choice: the lane exchange when
-// G >= N, else the full matrix while it is at most 16 KiB per orbit, else
-// none.  Measured (MI355X): the lane exchange runs N = 64 RK4 (16384 orbits)
-// 1.77x faster than the per-component form and N = 32 EM 1.11x faster than
-// the full matrix, while LDS matrices above 16 KiB per orbit limit the
-// kernel to about one wavefront per SIMD and lose to the per-component form.
-// Layouts 1 and 2 sum each row in j order (the per-component form's bits),
-// layout 3 in offset order (the same terms, the sum rounded differently).
+// exchange without storage (G >= N).  Options: -1 never, 1 always (the
+// automatic layout, else half storage), 2 half, 3 full, 4 lane exchange
+// (when G >= N, else none).  Automatic choice:
+//   G >= 2N and the full matrix at most 16 KiB per orbit: the full matrix,
+//     whose pair evaluations spread over all G lanes (small launches, where
+//     the group size is raised to use the spare lanes);
+//   G >= N: the lane exchange;
+//   else the full matrix while it is at most 16 KiB per orbit, else none.
+// Measured (MI355X): the lane exchange runs N = 64 RK4 (16384 orbits) 1.77x
+// faster than the per-component form and N = 32 EM 1.11x faster than the
+// full matrix, while LDS matrices above 16 KiB per orbit limit the kernel to
+// about one wavefront per SIMD and lose to the per-component form.  Layouts
+// 1 and 2 sum each row in j order (the per-component form's bits), layout 3
+// in offset order (the same terms, the sum rounded differently).
 int pairs_layout(const sodecl_ctx* c, int group)
 {
     if (!c->features.pairs || c->opts.pairwise < 0) return 0;
-    const bool exchange = group >= c->neq;
+    const bool full_fits = pairs_full_bytes(c) <= 16 * 1024;
+    const int automatic = (group >= 2 * c->neq && full_fits) ? 1 : group >= c->neq ? 3 : full_fits ? 1 : 0;
     switch (c->opts.pairwise) {
-    case 1: return exchange ? 3 : 2;
+    case 1: return automatic ? automatic : 2;
     case 2: return 2;
     case 3: return 1;
-    case 4: return exchange ? 3 : 0;
-    default: return exchange ? 3 : (pairs_full_bytes(c) <= 16 * 1024 ? 1 : 0);
+    case 4: return group >= c->neq ? 3 : 0;
+    default: return automatic;
     }
 }
 
@@ -292,6 +297,12 @@ LaunchCfg choose_config(const sodecl_ctx* c)
         cfg.block = std::min(b, 1024);
     } else if (sched == 2) {
         int g = c->opts.group_size > 0 ? c->opts.group_size : default_group(c->neq);
+        if (c->opts.group_size <= 0 && c->features.pairs && c->opts.pairwise >= 0) {
+            // a launch of less than one wavefront per SIMD: more lanes per
+            // orbit share its pair evaluations (up to a wavefront per orbit)
+            const long one_per_simd = 64L * 4 * device_cus(c);
+            while (g < 64 && (long)c->orbits * g * 2 <= one_per_simd) g *= 2;
+        }
         g = std::min(std::max(g, 1), 64);
         int p2 = 1;
         while (p2 < g) p2 <<= 1;

@@ -172,12 +172,12 @@ def test_reference_pybind_binding_builds_against_sodeclmgr(tmp_path):
 
 def test_pairwise_form_selected_and_lds_bounded():
     """Kuramoto defines sode_pair/sode_system_pairsum: the group schedule uses
-    the pairwise form automatically, by lane exchange (layout 3, no LDS)
-    when the group holds one component per lane, else with the full
-    N x (N+1) LDS matrix while that is at most 16 KiB per orbit; half
-    storage, the full matrix and the lane exchange when asked for, never
-    when disabled; blocks stay small enough that their LDS storage fits in
-    48 KiB."""
+    the pairwise form automatically: the full N x (N+1) LDS matrix when the
+    group has at least two lanes per component and the matrix is at most
+    16 KiB per orbit, by lane exchange (layout 3, no LDS) when it has one,
+    else the full matrix while it fits in 16 KiB; half storage, the full
+    matrix and the lane exchange when asked for, never when disabled;
+    blocks stay small enough that their LDS storage fits in 48 KiB."""
     path = sodecl_amd.system_path("kuramoto")
 
     def layout(n, g, mode):
@@ -194,7 +194,8 @@ def test_pairwise_form_selected_and_lds_bounded():
 
     # (N, G, automatic layout, layout for "always", layout for "exchange")
     for n, g, auto, always, exch in ((64, 64, 3, 3, 3), (32, 32, 3, 3, 3), (10, 16, 3, 3, 3), (45, 64, 3, 3, 3),
-                                     (64, 16, 0, 2, 0), (32, 16, 1, 2, 0), (44, 32, 1, 2, 0), (45, 32, 0, 2, 0)):
+                                     (64, 16, 0, 2, 0), (32, 16, 1, 1, 0), (44, 32, 1, 1, 0), (45, 32, 0, 2, 0),
+                                     (10, 32, 1, 1, 3), (10, 64, 1, 1, 3), (32, 64, 1, 1, 3)):
         assert layout(n, g, 0) == auto, (n, g)
         assert layout(n, g, 1) == always, (n, g)
         assert layout(n, g, 4) == exch, (n, g)
@@ -209,7 +210,8 @@ def test_pairwise_form_selected_and_lds_bounded():
 def test_group_blocks_fill_the_chip():
     """Group-schedule blocks shrink (down to one wavefront) while a launch
     would give fewer blocks than the 256 compute units; an explicit block
-    size is kept."""
+    size is kept.  With the pairwise drift a launch of less than a wavefront
+    per SIMD first widens its groups."""
     path = sodecl_amd.system_path("kuramoto")
 
     def cfg(orbits, n, **opts):
@@ -217,12 +219,17 @@ def test_group_blocks_fill_the_chip():
         s.compile_only("gfx950")
         return s.launch_config()
 
-    assert cfg(1024, 10) == (2, 16, 64)
-    assert cfg(2048, 10) == (2, 16, 128)
+    assert cfg(1024, 10, group_size=16) == (2, 16, 64)
+    assert cfg(2048, 10, group_size=16) == (2, 16, 128)
     assert cfg(4096, 10) == (2, 16, 256)
     assert cfg(16384, 64) == (2, 64, 256)
-    assert cfg(1024, 10, block_size=256) == (2, 16, 256)
+    assert cfg(1024, 10, group_size=16, block_size=256) == (2, 16, 256)
     assert cfg(16, 64) == (2, 64, 64)
+    # with the pairwise drift, small launches widen the group (up to a
+    # wavefront per orbit) so more lanes share the pair evaluations
+    assert cfg(1024, 10) == (2, 64, 256)
+    assert cfg(2048, 10) == (2, 32, 256)
+    assert cfg(1024, 10, pairwise=-1) == (2, 16, 64)
 
 
 def test_team_schedule_selection():

@@ -72,12 +72,14 @@ def test_graph_replay_matches_eager_launches():
     assert_traj_close(traj_g.cpu().numpy(), ref, what="graph")
 
 
-@pytest.mark.parametrize("solver,n,group", [(1, 7, 8), (2, 10, 4), (2, 32, 32), (2, 64, 64), (0, 32, 16), (0, 5, 8)])
+@pytest.mark.parametrize("solver,n,group", [(1, 7, 8), (2, 10, 4), (2, 32, 32), (2, 64, 64), (0, 32, 16), (0, 5, 8),
+                                            (1, 10, 64), (2, 7, 32), (0, 6, 32)])
 def test_pairwise_drift_bit_identical(solver, n, group):
     """The pairwise form of the Kuramoto drift (each sin(y_j - y_i) evaluated
     once, its negation mirrored), in both LDS layouts, sums the same terms
     in the same order as sode_system_i, so trajectories must agree bit for
-    bit."""
+    bit -- also with groups wider than the system, whose spare lanes
+    evaluate pairs too."""
     orbits = 301
     if solver == 0:
         cfg = configs.kuramoto_em(orbits=orbits, n=n, seed=n, tspan=0.2, ksteps=10)
