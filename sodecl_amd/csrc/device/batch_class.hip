@@ -50,11 +50,14 @@ SODECL_INLINE double sodecl_rcp(double b)
     return r;
 }
 
-SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up, double identity)
+// v of lane - s (up) or lane + s (down), 0 past either end of the wavefront;
+// a distance of one is a DPP wavefront shift, longer ones a bpermute.
+SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
 {
+    if (s == 1) return up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v) : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v);
     const double w = up ? __shfl_up(v, (unsigned)s, SODECL_BG) : __shfl_down(v, (unsigned)s, SODECL_BG);
     const bool inside = up ? lane - s >= 0 : lane + s < SODECL_BG;
-    return inside ? w : identity;
+    return inside ? w : 0.0;
 }
 
 // Partitions that hold rows, and the cyclic-reduction levels they need
@@ -122,9 +125,9 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     const double e_first = has_in ? e[0] : 0.0;
     const double f_first = has_in ? f[0] : 0.0;
     const double g_first = has_in ? g[0] : 1.0;   // without interior its first row is its X
-    const double e_nx = __shfl_down(e_first, 1u, SODECL_BG);
-    const double f_nx = __shfl_down(f_first, 1u, SODECL_BG);
-    const double g_nx = __shfl_down(g_first, 1u, SODECL_BG);
+    const double e_nx = sodecl_pcr_neighbour(e_first, lane, 1, false);
+    const double f_nx = sodecl_pcr_neighbour(f_first, lane, 1, false);
+    const double g_nx = sodecl_pcr_neighbour(g_first, lane, 1, false);
     double A = 0.0, B = 1.0, C = 0.0, D = 0.0;
     if (active) {
         const double ar = sodecl_tri_a(up, r1);   // zero for row 0
@@ -135,33 +138,33 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
         C = has_next ? cr * g_nx : 0.0;
         D = v[SODECL_TI(r1)] - ar * e_last - (has_next ? cr * e_nx : 0.0);
     }
-    // parallel cyclic reduction over the reduced rows; each lane shares
-    // its pivot's reciprocal so one reciprocal per lane and level suffices
+    // parallel cyclic reduction over the reduced rows.  Each lane shares its
+    // row scaled by its pivot's reciprocal (A/B, C/B, D/B), so one
+    // reciprocal per lane and level and three values per direction suffice:
+    // with the neighbours m = lane - s and p = lane + s,
+    //   A' = -A (A/B)_m,  C' = -C (C/B)_p,
+    //   B' = B - A (C/B)_m - C (A/B)_p,  D' = D - A (D/B)_m - C (D/B)_p;
+    // rows past either end contribute nothing (shared values 0).
 #pragma unroll
     for (int lv = 0; lv < SODECL_PCR_LEVELS; ++lv) {
         const int s = 1 << lv;
         const double iB = sodecl_rcp(B);
-        const double Am = sodecl_pcr_neighbour(A, lane, s, true, 0.0);
-        const double iBm = sodecl_pcr_neighbour(iB, lane, s, true, 1.0);
-        const double Cm = sodecl_pcr_neighbour(C, lane, s, true, 0.0);
-        const double Dm = sodecl_pcr_neighbour(D, lane, s, true, 0.0);
-        const double Ap = sodecl_pcr_neighbour(A, lane, s, false, 0.0);
-        const double iBp = sodecl_pcr_neighbour(iB, lane, s, false, 1.0);
-        const double Cp = sodecl_pcr_neighbour(C, lane, s, false, 0.0);
-        const double Dp = sodecl_pcr_neighbour(D, lane, s, false, 0.0);
-        const double k1 = A * iBm;
-        const double k2 = C * iBp;
-        const double nA = -k1 * Am;
-        const double nC = -k2 * Cp;
-        const double nB = B - k1 * Cm - k2 * Ap;
-        const double nD = D - k1 * Dm - k2 * Dp;
-        A = nA;
+        const double a = A * iB, c = C * iB, d = D * iB;
+        const double am = sodecl_pcr_neighbour(a, lane, s, true);
+        const double cm = sodecl_pcr_neighbour(c, lane, s, true);
+        const double dm = sodecl_pcr_neighbour(d, lane, s, true);
+        const double ap = sodecl_pcr_neighbour(a, lane, s, false);
+        const double cp = sodecl_pcr_neighbour(c, lane, s, false);
+        const double dp = sodecl_pcr_neighbour(d, lane, s, false);
+        const double nB = B - A * cm - C * ap;
+        const double nD = D - A * dm - C * dp;
+        A = -A * am;
+        C = -C * cp;
         B = nB;
-        C = nC;
         D = nD;
     }
     const double X = D * sodecl_rcp(B);
-    const double Xm = sodecl_pcr_neighbour(X, lane, 1, true, 0.0);
+    const double Xm = sodecl_pcr_neighbour(X, lane, 1, true);
     if (active) {
 #pragma unroll
         for (int j = 0; j < SODECL_BR - 1; ++j)
@@ -185,8 +188,16 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
     double* const sh_lo = sh_r + SODECL_TLEN;
     double* const sh_di = sh_lo + SODECL_TLEN;
     double* const sh_up = sh_di + SODECL_TLEN;
+#if _numpar_ <= 8
+    // few parameters: a register copy, which the plugin's reads need not
+    // reload from LDS after every store of the band rows
+    double sh_p[_numpar_];
+#pragma unroll
+    for (int k = 0; k < _numpar_; ++k) sh_p[k] = prm[k];
+#else
     double* const sh_p = sh_up + SODECL_TLEN;
     for (int k = lane; k < _numpar_; k += SODECL_BG) sh_p[k] = prm[k];
+#endif
     double y[SODECL_BR], gg[SODECL_BR];
 #pragma unroll
     for (int c = 0; c < SODECL_BR; ++c) {

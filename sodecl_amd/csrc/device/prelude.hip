@@ -15,6 +15,8 @@ __device__ inline double sodecl_dpp(double v)
 #define SODECL_DPP_XOR3 0x1B          // quad_perm [3,2,1,0]
 #define SODECL_DPP_HALF_MIRROR 0x141  // lane i <- 7 - i within each 8 lanes
 #define SODECL_DPP_MIRROR 0x140       // lane i <- 15 - i within each 16 lanes
+#define SODECL_DPP_WAVE_SHR1 0x138    // lane i <- lane i - 1 across the wavefront (lane 0 <- 0)
+#define SODECL_DPP_WAVE_SHL1 0x130    // lane i <- lane i + 1 across the wavefront (lane 63 <- 0)
 
 // Under the group schedule SODECL_GROUP consecutive lanes of a wavefront
 // share one orbit.  A plugin entry point that the whole group calls together
