@@ -51,10 +51,13 @@ SODECL_INLINE double sodecl_rcp(double b)
 }
 
 // v of lane - s (up) or lane + s (down), 0 past either end of the wavefront;
-// a distance of one is a DPP wavefront shift, longer ones a bpermute.
+// distances of one and two are DPP wavefront shifts, longer ones a bpermute.
 SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
 {
     if (s == 1) return up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v) : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v);
+    if (s == 2)
+        return up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v))
+                  : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v));
     const double w = up ? __shfl_up(v, (unsigned)s, SODECL_BG) : __shfl_down(v, (unsigned)s, SODECL_BG);
     const bool inside = up ? lane - s >= 0 : lane + s < SODECL_BG;
     return inside ? w : 0.0;
