@@ -36,6 +36,8 @@
 
 #define SODECL_NPAR_ALLOC (_numpar_ > 0 ? _numpar_ : 1)
 #define SODECL_NNOI_ALLOC (_numnoi_ > 0 ? _numnoi_ : 1)
+// Threefry calls per step: one per pair of noise values
+#define SODECL_CALLS ((_numnoi_ + 1) / 2)
 #define SODECL_IS_SDE (SODECL_SOLVER == 0)
 #define SODECL_IS_IMPLICIT (SODECL_SOLVER == 3 || SODECL_SOLVER == 4)
 
@@ -701,6 +703,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
     double cnt = active ? counters[orbit] : 0.0;
     const double dtsqrt = sqrt(_m_dt_);
     const int calls = (int)ceilf((float)((float)(_numnoi_) * 0.5));
+    bool noise_ready = false;   // this step's noise was drawn with the previous step's
 #endif
     int buf = 0;
     long long g = gstep0;
@@ -733,11 +736,24 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
             // Noise: Threefry call k of this step goes to lane k of the group;
             // the counter of call k is counter + k, as the reference's
             // sequential +1 per call (stochastic_solver_caller.cl:44-66).
-            for (int k = lane; k < calls; k += SODECL_G) {
-                double z0, z1;
-                sodecl_noise_pair(orbit, cnt + (double)k, dtsqrt, &z0, &z1);
-                if (2 * k < _numnoi_) sh_n[buf][slot][2 * k] = z0;
-                if (2 * k + 1 < _numnoi_) sh_n[buf][slot][2 * k + 1] = z1;
+            // The noise does not depend on the state: a group with at least
+            // twice as many lanes as calls draws the next step of the same
+            // segment in the same pass (lanes calls .. 2 calls - 1, counters
+            // counter + calls + k, into the next step's buffer), and that
+            // step draws none.
+            if (!noise_ready) {
+                const int n = (SODECL_G >= 2 * SODECL_CALLS && it + 1 < _numsteps_) ? 2 * calls : calls;
+                for (int k = lane; k < n; k += SODECL_G) {
+                    const int nb = k < calls ? buf : (buf ^ 1);
+                    const int kk = k < calls ? k : k - calls;
+                    double z0, z1;
+                    sodecl_noise_pair(orbit, cnt + (double)k, dtsqrt, &z0, &z1);
+                    if (2 * kk < _numnoi_) sh_n[nb][slot][2 * kk] = z0;
+                    if (2 * kk + 1 < _numnoi_) sh_n[nb][slot][2 * kk + 1] = z1;
+                }
+                noise_ready = n > calls;
+            } else {
+                noise_ready = false;
             }
             cnt = cnt + (double)calls;
             sodecl_publish(sh_x[buf][slot], y, lane);

@@ -127,9 +127,12 @@ def test_kuramoto_group_vs_oracle(n, solver, group):
     assert_traj_close(lane, ref["out"], what="lane")
 
 
-@pytest.mark.parametrize("n,group", [(3, 4), (32, 32), (32, 16), (5, 8)])
-def test_kuramoto_em_vs_oracle(n, group):
-    cfg = configs.kuramoto_em(orbits=150, n=n, seed=7, tspan=0.2, ksteps=20)
+@pytest.mark.parametrize("n,group,ksteps", [(3, 4, 20), (32, 32, 20), (32, 16, 20), (5, 8, 20), (5, 8, 7), (3, 64, 9)])
+def test_kuramoto_em_vs_oracle(n, group, ksteps):
+    """Groups with at least twice as many lanes as Threefry calls draw two
+    steps of noise per pass; an odd segment length leaves its last step to
+    draw its own (counters must still advance by one per call)."""
+    cfg = configs.kuramoto_em(orbits=150, n=n, seed=7, tspan=0.2, ksteps=ksteps)
     rng = np.random.default_rng(3)
     cfg["counters"] = rng.integers(0, 2 ** 40, cfg["orbits"]).astype(np.float64)
     s, out = run_session(cfg, schedule="group", group_size=group)
