@@ -473,14 +473,14 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
     (void)S;
     const int i = lane;
     const bool row = i < _numeq_;
+    const int ir = row ? i : 0;   // lanes past the last row index row 0 and discard the result
     double acc = 0.0;
     for (int d = 1; d <= SODECL_HALF; ++d) {
-        int j = i + d;
+        int j = ir + d;
         if (j >= _numeq_) j -= _numeq_;
-        int src = i - d;
+        int src = ir - d;
         if (src < 0) src += _numeq_;
         const double v = row ? sode_pair(i, j, t, x, pp) : 0.0;
-        // lanes past the last row read some lane of the group and discard it
         const double w = __shfl(v, src, SODECL_G);
         acc = acc + v;
         acc = acc - w;
@@ -490,7 +490,7 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
         // negation from row i - N/2
         const int h = _numeq_ / 2;
         const double v = (i < h) ? sode_pair(i, i + h, t, x, pp) : 0.0;
-        const double w = __shfl(v, i >= h ? i - h : i, SODECL_G);
+        const double w = __shfl(v, (row && i >= h) ? i - h : 0, SODECL_G);
         acc = i < h ? acc + v : acc - w;
     }
     f[0] = row ? sode_system_pairsum(i, t, x, pp, acc) : 0.0;

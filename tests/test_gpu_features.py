@@ -100,13 +100,14 @@ def test_pairwise_drift_bit_identical(solver, n, group):
 
 
 @pytest.mark.parametrize("solver,n,group", [(1, 7, 8), (2, 32, 32), (2, 64, 64), (2, 33, 64), (2, 3, 4),
-                                            (0, 5, 8), (0, 32, 32)])
+                                            (0, 5, 8), (0, 32, 32), (1, 10, 64), (0, 6, 32)])
 def test_pairwise_lane_exchange_within_tolerance(solver, n, group):
     """The lane-exchange pairwise form (one component per lane, no LDS
     matrix) sums each row's terms in offset order rather than j order: the
     drift is the per-component drift up to the rounding of that sum, so
     trajectories agree with the per-component form and with the oracle at
-    the FP64 parity tolerance, and EM noise counters exactly."""
+    the FP64 parity tolerance, and EM noise counters exactly -- also in
+    groups with lanes to spare past the last row."""
     orbits = 301
     if solver == 0:
         cfg = configs.kuramoto_em(orbits=orbits, n=n, seed=n, tspan=0.2, ksteps=10)
