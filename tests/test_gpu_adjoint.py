@@ -42,14 +42,20 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("solver", [1, 2])
-@pytest.mark.parametrize("n,group", [(6, 8), (16, 16), (20, 8), (64, 64), (32, 32), (3, 4), (5, 2)])
+@pytest.mark.parametrize("n,group", [(6, 8), (16, 16), (20, 8), (64, 64), (32, 32), (3, 4), (5, 2), (6, 64), (10, 0)])
 def test_adjoint_vs_forward_sensitivity(solver, n, group):
+    """Group sizes cover every pairwise layout: lane exchange (N <= G < 2N),
+    the LDS matrix with its pairs spread over spare lanes (G >= 2N, and the
+    group widened automatically for this small launch when G is 0), and
+    the LDS matrix for groups narrower than N."""
     orbits = 37
     cfg, integ, y0, p = make(n, orbits, solver, seed=n, group=group)
     rng = np.random.default_rng(n)
     cot_np = rng.normal(size=(integ.nsnap, n, orbits))
     cot = torch.tensor(cot_np, device=DEV)
     gy, gp = grads(integ, y0, p, cot)
+    if group == 0:
+        assert integ.session.launch_config()[1] == 64
     k = 5 if n == 64 else orbits
     ry, rp = Oracle("kuramoto").sensitivity_gradient({1: "euler", 2: "rk4"}[solver], cfg["y0"], cfg["params"], cot_np,
                                                      cfg["dt"], cfg["tspan"], cfg["ksteps"], orbit_range=(0, k))
@@ -215,7 +221,7 @@ def test_implicit_adjoint_checkpoint_interval_and_default_threshold():
         assert np.array_equal(gy, res[0][0]) and np.array_equal(gp, res[0][1])
 
 
-@pytest.mark.parametrize("solver,n,group", [(1, 16, 16), (2, 64, 64), (2, 20, 8)])
+@pytest.mark.parametrize("solver,n,group", [(1, 16, 16), (2, 64, 64), (2, 20, 8), (2, 10, 64)])
 def test_stored_stage_adjoint_bit_identical_to_checkpointed(solver, n, group):
     """The adjoint without recomputation (the forward pass stores every
     step's stage vectors) gives the checkpointed adjoint's gradients and the
