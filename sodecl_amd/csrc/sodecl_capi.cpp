@@ -135,7 +135,7 @@ std::shared_ptr<Program> get_program(sodecl_ctx* ctx, int schedule, int group, i
     ctx->last_program = text;
     auto prog = std::make_shared<Program>();
     std::string log;
-    if (!compile_program(text, device_arch(ctx->device), key.fp_contract, prog->code, log)) {
+    if (!compile_program(text, device_arch(ctx->device), key.fp_contract, prog->code, log, key.one_wave_per_simd)) {
         // the reference prints the build log on failure (sodeclmgr.cpp:606-619)
         fail(ctx, "Failed to build program executable:\n" + log);
         return nullptr;
@@ -1058,7 +1058,8 @@ size_t sodecl_compile_only(sodecl_ctx* c, const char* arch)
         c->last_program = assemble_program(c->system_source, key, c->features);
         std::vector<char> code;
         std::string log;
-        if (!compile_program(c->last_program, arch ? arch : "gfx950", key.fp_contract, code, log)) {
+        if (!compile_program(c->last_program, arch ? arch : "gfx950", key.fp_contract, code, log,
+                             key.one_wave_per_simd)) {
             fail(c, "Failed to build program executable:\n" + log);
             return 0;
         }

@@ -70,8 +70,10 @@ struct Program {
 
 // Compiles (through an in-memory and on-disk cache keyed by the program
 // text, the options and the architecture) and returns the code object.
+// ilp_schedule: schedule instructions for instruction-level parallelism
+// (programs whose launches have at most one wavefront per SIMD).
 bool compile_program(const std::string& text, const std::string& arch, int fp_contract, std::vector<char>& code,
-                     std::string& log);
+                     std::string& log, bool ilp_schedule = false);
 std::string device_arch(int device);
 
 long num_launches(double tspan, double dt, int ksteps);
