@@ -1,7 +1,7 @@
 // Device helpers visible to the system plugin (emitted before it).
 
-// A double moved across lanes by one DPP control (two 32-bit moves; all
-// source lanes of the patterns used here lie in the same 16-lane row).
+// A double moved across lanes by one DPP control (two 32-bit moves); a lane
+// whose source lies outside the wavefront receives 0.
 template <int CTRL>
 __device__ inline double sodecl_dpp(double v)
 {
@@ -36,14 +36,26 @@ __device__ inline int sodecl_lane() { return (int)(threadIdx.x % SODECL_GROUP); 
 // partial sum) exactly as xor 4 and xor 8 would; the last two are cross-lane
 // permutes.  At every level the two partners add the same two numbers, so
 // every lane ends with a bit-identical sum.
+// A group of the whole wavefront reads the four row sums into scalar
+// registers instead (v_readlane) and adds them as (r0 + r1) + (r2 + r3):
+// the same two levels of additions on the same operands, without the
+// permutes.
+__device__ inline double sodecl_readlane(double v, int lane)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
 __device__ inline double sodecl_group_sum(double v)
 {
     if (SODECL_GROUP >= 2) v += sodecl_dpp<SODECL_DPP_XOR1>(v);
     if (SODECL_GROUP >= 4) v += sodecl_dpp<SODECL_DPP_XOR2>(v);
     if (SODECL_GROUP >= 8) v += sodecl_dpp<SODECL_DPP_HALF_MIRROR>(v);
     if (SODECL_GROUP >= 16) v += sodecl_dpp<SODECL_DPP_MIRROR>(v);
+    if (SODECL_GROUP == 64)
+        return (sodecl_readlane(v, 0) + sodecl_readlane(v, 16)) + (sodecl_readlane(v, 32) + sodecl_readlane(v, 48));
     if (SODECL_GROUP >= 32) v += __shfl_xor(v, 16, SODECL_GROUP);
-    if (SODECL_GROUP >= 64) v += __shfl_xor(v, 32, SODECL_GROUP);
     return v;
 }
 #define SODECL_GROUP_API 1
