@@ -50,6 +50,21 @@ SODECL_INLINE double sodecl_rcp(double b)
     return r;
 }
 
+// The values of lanes l - 32 (up) and l + 32 (down), 0 past either end of
+// the wavefront, from one half-wavefront swap (v_permlane32_swap) per 32-bit
+// half: with both operands v, its first result holds lane l mod 32's value
+// and its second lane 32 + l mod 32's.
+SODECL_INLINE void sodecl_pcr_pair32(double v, int lane, double& up, double& down)
+{
+    const long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((int)(b & 0xffffffffll), (int)(b & 0xffffffffll), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((int)(b >> 32), (int)(b >> 32), false, false);
+    const double from_low = __longlong_as_double(((long long)(int)hi[0] << 32) | (long long)(unsigned int)lo[0]);
+    const double from_high = __longlong_as_double(((long long)(int)hi[1] << 32) | (long long)(unsigned int)lo[1]);
+    up = lane >= 32 ? from_low : 0.0;
+    down = lane < 32 ? from_high : 0.0;
+}
+
 // v of lane - s (up) or lane + s (down), 0 past either end of the wavefront;
 // distances of one and two are DPP wavefront shifts, longer ones a bpermute.
 SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
@@ -153,12 +168,19 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
         const int s = 1 << lv;
         const double iB = sodecl_rcp(B);
         const double a = A * iB, c = C * iB, d = D * iB;
-        const double am = sodecl_pcr_neighbour(a, lane, s, true);
-        const double cm = sodecl_pcr_neighbour(c, lane, s, true);
-        const double dm = sodecl_pcr_neighbour(d, lane, s, true);
-        const double ap = sodecl_pcr_neighbour(a, lane, s, false);
-        const double cp = sodecl_pcr_neighbour(c, lane, s, false);
-        const double dp = sodecl_pcr_neighbour(d, lane, s, false);
+        double am, cm, dm, ap, cp, dp;
+        if (s == 32) {
+            sodecl_pcr_pair32(a, lane, am, ap);
+            sodecl_pcr_pair32(c, lane, cm, cp);
+            sodecl_pcr_pair32(d, lane, dm, dp);
+        } else {
+            am = sodecl_pcr_neighbour(a, lane, s, true);
+            cm = sodecl_pcr_neighbour(c, lane, s, true);
+            dm = sodecl_pcr_neighbour(d, lane, s, true);
+            ap = sodecl_pcr_neighbour(a, lane, s, false);
+            cp = sodecl_pcr_neighbour(c, lane, s, false);
+            dp = sodecl_pcr_neighbour(d, lane, s, false);
+        }
         const double nB = B - A * cm - C * ap;
         const double nD = D - A * dm - C * dp;
         A = -A * am;
