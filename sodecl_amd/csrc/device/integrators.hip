@@ -476,10 +476,16 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
     const int ir = row ? i : 0;   // lanes past the last row index row 0 and discard the result
     double acc = 0.0;
     for (int d = 1; d <= SODECL_HALF; ++d) {
-        int j = ir + d;
-        if (j >= _numeq_) j -= _numeq_;
-        int src = ir - d;
-        if (src < 0) src += _numeq_;
+        int j, src;
+        if ((_numeq_ & (_numeq_ - 1)) == 0) {   // N a power of two: wrap by mask
+            j = (ir + d) & (_numeq_ - 1);
+            src = (ir - d) & (_numeq_ - 1);
+        } else {
+            j = ir + d;
+            if (j >= _numeq_) j -= _numeq_;
+            src = ir - d;
+            if (src < 0) src += _numeq_;
+        }
         const double v = row ? sode_pair(i, j, t, x, pp) : 0.0;
         const double w = __shfl(v, src, SODECL_G);
         acc = acc + v;
