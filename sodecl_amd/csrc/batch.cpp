@@ -24,14 +24,15 @@ struct sodecl_batch {
     int device = 0;
     // per-orbit layout
     int orbits = 0;
-    std::vector<int> cls, steps, order;
+    std::vector<int> cls, steps, order, items;   // items: pairs of orbits (-1: none), longest first
+    int nitems = 0;
     std::vector<long long> yoff, ooff;
     size_t state_size = 0, output_size = 0;
     // device
     std::vector<char> code;
     hipModule_t module = nullptr;
     hipFunction_t fn = nullptr, fn_reset = nullptr;
-    int *d_order = nullptr, *d_cls = nullptr, *d_steps = nullptr, *d_head = nullptr;
+    int *d_items = nullptr, *d_cls = nullptr, *d_steps = nullptr, *d_head = nullptr;
     long long *d_yoff = nullptr, *d_ooff = nullptr;
     unsigned grid = 0;
     size_t lds_bytes = 0;
@@ -56,7 +57,7 @@ bool bhip(sodecl_batch* b, hipError_t e, const char* what)
 void free_batch(sodecl_batch* b)
 {
     if (b->module) hipModuleUnload(b->module);
-    hipFree(b->d_order);
+    hipFree(b->d_items);
     hipFree(b->d_cls);
     hipFree(b->d_steps);
     hipFree(b->d_head);
@@ -64,7 +65,7 @@ void free_batch(sodecl_batch* b)
     hipFree(b->d_ooff);
     b->module = nullptr;
     b->fn = b->fn_reset = nullptr;
-    b->d_order = b->d_cls = b->d_steps = b->d_head = nullptr;
+    b->d_items = b->d_cls = b->d_steps = b->d_head = nullptr;
     b->d_yoff = b->d_ooff = nullptr;
 }
 
@@ -160,14 +161,12 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     b->ooff.resize(orbits);
     size_t ys = 0, os = 0;
     std::vector<double> cost(orbits);
-    int nmax = 0;
     for (int o = 0; o < orbits; ++o) {
         const int c = b->cls[o];
         if (c < 0 || c >= (int)b->class_neq.size()) return bfail(b, "orbit size class out of range");
         if (b->steps[o] < 0 || b->steps[o] % b->ksteps != 0)
             return bfail(b, "per-orbit step counts must be non-negative multiples of ksteps");
         const int n = b->class_neq[c];
-        nmax = std::max(nmax, n);
         b->yoff[o] = (long long)ys;
         b->ooff[o] = (long long)os;
         ys += n;
@@ -180,6 +179,29 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     b->order.resize(orbits);
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    // work items: one orbit, or two consecutive orbits of the same class
+    // (similar costs) where that class runs 32 lanes per orbit
+    const int nclass = (int)b->class_neq.size();
+    std::vector<std::vector<int>> per_class(nclass);
+    for (int o : b->order) per_class[b->cls[o]].push_back(o);
+    struct Item {
+        double cost;
+        int o0, o1;
+    };
+    std::vector<Item> items;
+    for (int c = 0; c < nclass; ++c) {
+        const int opw = 64 / batch_lanes_per_orbit(b->class_neq[c]);
+        const auto& v = per_class[c];
+        for (size_t i = 0; i < v.size(); i += opw)
+            items.push_back({cost[v[i]], v[i], (opw == 2 && i + 1 < v.size()) ? v[i + 1] : -1});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) { return a.cost > c.cost; });
+    b->nitems = (int)items.size();
+    b->items.resize(2 * items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        b->items[2 * i] = items[i].o0;
+        b->items[2 * i + 1] = items[i].o1;
+    }
 
     if (!bhip(b, hipSetDevice(b->device), "hipSetDevice")) return 0;
     b->program = assemble_batch_program(b->source, batch_key(b), b->features, b->class_neq);
@@ -190,9 +212,19 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     if (!bhip(b, hipModuleGetFunction(&b->fn, b->module, "sodecl_batch"), "sodecl_batch")) return 0;
     if (!bhip(b, hipModuleGetFunction(&b->fn_reset, b->module, "sodecl_batch_reset"), "sodecl_batch_reset"))
         return 0;
-    // plain evaluation row + five padded solve rows (one pad per partition) + parameters
-    // evaluation point, four padded rows (residual/solution, band), parameters (batch_class.hip)
-    b->lds_bytes = (size_t)(nmax + 4 * (nmax + 64) + std::max(b->npar, 1)) * sizeof(double);
+    // LDS per wavefront: per orbit the evaluation point, four padded solve
+    // rows and (more than 8) parameters, times the orbits per wavefront
+    // (batch_class.hip); the largest class sets the launch's size
+    size_t lds_doubles = 1;
+    for (int c = 0; c < nclass; ++c) {
+        const long n = b->class_neq[c];
+        const long lpo = batch_lanes_per_orbit((int)n);
+        const long br = (n + lpo - 1) / lpo;
+        const long tlen = br >= 2 ? n + lpo : n;
+        const long per_orbit = n + 4 * tlen + (b->npar <= 8 ? 0 : b->npar);
+        lds_doubles = std::max(lds_doubles, (size_t)(per_orbit * (64 / lpo)));
+    }
+    b->lds_bytes = lds_doubles * sizeof(double);
     int per_cu = 0;
     if (!bhip(b, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, 64, b->lds_bytes),
               "occupancy"))
@@ -200,12 +232,12 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     int cus = 0;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
     long g = (long)std::max(per_cu, 1) * std::max(cus, 1);
-    b->grid = (unsigned)std::min<long>(g, orbits);
+    b->grid = (unsigned)std::min<long>(g, b->nitems);
     auto up = [&](void** dst, const void* src, size_t bytes, const char* what) {
         return bhip(b, hipMalloc(dst, bytes), what) &&
                bhip(b, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
     };
-    if (!up((void**)&b->d_order, b->order.data(), orbits * sizeof(int), "order")) return 0;
+    if (!up((void**)&b->d_items, b->items.data(), b->items.size() * sizeof(int), "work items")) return 0;
     if (!up((void**)&b->d_cls, b->cls.data(), orbits * sizeof(int), "classes")) return 0;
     if (!up((void**)&b->d_steps, b->steps.data(), orbits * sizeof(int), "steps")) return 0;
     if (!up((void**)&b->d_yoff, b->yoff.data(), orbits * sizeof(long long), "state offsets")) return 0;
@@ -235,8 +267,8 @@ int sodecl_batch_run_device(sodecl_batch* b, const double* y0, const double* par
     hipStream_t s = (hipStream_t)stream;
     void* rargs[] = {&b->d_head};
     if (!bhip(b, hipModuleLaunchKernel(b->fn_reset, 1, 1, 1, 1, 1, 1, 0, s, rargs, nullptr), "reset")) return 0;
-    int nitems = b->orbits;
-    void* args[] = {&b->d_order, &b->d_cls, &b->d_steps, &b->d_yoff, &b->d_ooff, (void*)&y0, (void*)&params,
+    int nitems = b->nitems;
+    void* args[] = {&b->d_items, &b->d_cls, &b->d_steps, &b->d_yoff, &b->d_ooff, (void*)&y0, (void*)&params,
                     &out, &newton, &b->d_head, &nitems};
     return bhip(b, hipModuleLaunchKernel(b->fn, b->grid, 1, 1, 64, 1, 1, (unsigned)b->lds_bytes, s, args, nullptr),
                 "hipModuleLaunchKernel(batch)")

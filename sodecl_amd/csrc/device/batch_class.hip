@@ -1,15 +1,17 @@
 // Per-size-class part of the batched implicit integrator.  The host emits
 // this text once per system size class, inside a namespace, after defining
-// _numeq_ for that class and including the system plugin there.
+// _numeq_ and SODECL_LPO (lanes per orbit: 64, or 32 for classes of at most
+// 128 equations, two orbits per wavefront) for that class and including the
+// system plugin there.
 //
-// One orbit is integrated by one wavefront (SODECL_BG = 64 lanes):
-//   * components are owned strided (comp = lane + 64 c) for the right-hand
+// One orbit is integrated by SODECL_LPO lanes of a wavefront:
+//   * components are owned strided (comp = lane + LPO c) for the right-hand
 //     side, the band Jacobian and the Newton update;
 //   * the tridiagonal solve of each Newton iteration runs in parallel over
 //     contiguous row partitions (lane k owns rows [kR, kR + R)): elimination
 //     inside each partition expresses its rows through the last unknown of
 //     the previous partition and its own last unknown, the resulting
-//     tridiagonal system in those 64 unknowns is solved by parallel cyclic
+//     tridiagonal system in those LPO unknowns is solved by parallel cyclic
 //     reduction across lanes, and the partition rows follow by substitution.
 // The matrix is the one the reference's tridiagonal sweep solves
 // (ie.cl:10-48, im.cl:3-41): sub-diagonal a_i = J[i][i+1] for i < N-2 (0
@@ -17,21 +19,26 @@
 // each entry already scaled to I + s*dt*J.  Solving it by partitions instead
 // of one sweep changes only the rounding.
 
-#define SODECL_BR ((_numeq_ + SODECL_BG - 1) / SODECL_BG)   // rows per partition == components per lane
+#define SODECL_BR ((_numeq_ + SODECL_LPO - 1) / SODECL_LPO)   // rows per partition == components per lane
+#define SODECL_OPW (SODECL_BG / SODECL_LPO)                   // orbits per wavefront
 // Every lane owns exactly SODECL_BR components / rows (neq a multiple of
-// 64): the bounds tests below are then compile-time true.
-#define SODECL_FULL ((_numeq_ % SODECL_BG) == 0)
+// LPO): the bounds tests below are then compile-time true.
+#define SODECL_FULL ((_numeq_ % SODECL_LPO) == 0)
 #define SODECL_OWNS(comp) (SODECL_FULL || (comp) < _numeq_)
 // LDS index of row i in the solve's arrays: one pad slot per partition, so
-// the 64 lanes reading row j of their own partitions (stride R + 1 doubles)
+// the lanes reading row j of their own partitions (stride R + 1 doubles)
 // spread over the LDS banks instead of colliding on a stride of R.
 #if SODECL_BR >= 2
 #define SODECL_TI(i) ((i) + (i) / SODECL_BR)
-#define SODECL_TLEN (_numeq_ + SODECL_BG)
+#define SODECL_TLEN (_numeq_ + SODECL_LPO)
 #else
 #define SODECL_TI(i) (i)
 #define SODECL_TLEN (_numeq_)
 #endif
+// LDS doubles per orbit: evaluation point, four padded solve rows, and the
+// parameters when there are more than 8 (fewer stay in registers)
+#define SODECL_PLDS (_numpar_ <= 8 ? 0 : _numpar_)
+#define SODECL_ORBIT_LDS (_numeq_ + 4 * SODECL_TLEN + SODECL_PLDS)
 
 // Coefficients of row i of the reference's tridiagonal matrix.
 SODECL_INLINE double sodecl_tri_a(const double* up, int i)
@@ -65,17 +72,36 @@ SODECL_INLINE void sodecl_pcr_pair32(double v, int lane, double& up, double& dow
     down = lane < 32 ? from_high : 0.0;
 }
 
-// v of lane - s (up) or lane + s (down), 0 past either end of the wavefront;
-// distances of one and two are DPP wavefront shifts, longer ones a bpermute.
+// v of orbit lane - s (up) or lane + s (down), 0 past either end of the
+// orbit's lanes; distances of one and two are DPP wavefront shifts (with a
+// whole wavefront per orbit the shift itself brings 0 past either end;
+// narrower orbits drop what crossed over from the neighbouring orbit),
+// longer ones a bpermute.
 SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
 {
-    if (s == 1) return up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v) : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v);
-    if (s == 2)
-        return up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v))
-                  : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v));
-    const double w = up ? __shfl_up(v, (unsigned)s, SODECL_BG) : __shfl_down(v, (unsigned)s, SODECL_BG);
-    const bool inside = up ? lane - s >= 0 : lane + s < SODECL_BG;
+    const bool inside = up ? lane - s >= 0 : lane + s < SODECL_LPO;
+    if (s <= 2) {
+        double w = up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v) : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v);
+        if (s == 2) w = up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(w) : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(w);
+        return (SODECL_LPO == 64 || inside) ? w : 0.0;
+    }
+    const double w = up ? __shfl_up(v, (unsigned)s, SODECL_LPO) : __shfl_down(v, (unsigned)s, SODECL_LPO);
     return inside ? w : 0.0;
+}
+
+// Sum over the orbit's lanes, the same value on each of them.
+SODECL_INLINE double sodecl_orbit_sum(double v)
+{
+#if SODECL_LPO == 64
+    return sodecl_group_sum(v);
+#else
+    v += sodecl_dpp<SODECL_DPP_XOR1>(v);
+    v += sodecl_dpp<SODECL_DPP_XOR2>(v);
+    v += sodecl_dpp<SODECL_DPP_HALF_MIRROR>(v);
+    v += sodecl_dpp<SODECL_DPP_MIRROR>(v);
+    v += __shfl_xor(v, 16, SODECL_LPO);
+    return v;
+#endif
 }
 
 // Partitions that hold rows, and the cyclic-reduction levels they need
@@ -86,7 +112,8 @@ SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
 
 // Solves M x = v for the matrix above; lo, di, up, v, x in LDS (padded
 // layout); x may be v (each lane reads only its own partition's rows of v,
-// all before it writes them); all 64 lanes.
+// all before it writes them); all lanes of the wavefront, lane = the lane's
+// index within its orbit.
 SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const double* up, const double* v, double* x,
                                     int lane)
 {
@@ -169,7 +196,7 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
         const double iB = sodecl_rcp(B);
         const double a = A * iB, c = C * iB, d = D * iB;
         double am, cm, dm, ap, cp, dp;
-        if (s == 32) {
+        if (SODECL_LPO == 64 && s == 32) {
             sodecl_pcr_pair32(a, lane, am, ap);
             sodecl_pcr_pair32(c, lane, cm, cp);
             sodecl_pcr_pair32(d, lane, dm, dp);
@@ -198,37 +225,56 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     }
 }
 
-// Integrates one orbit for `steps` steps of implicit Euler (SODECL_SOLVER 3,
-// ie.cl:50-117) or implicit midpoint (4, im.cl:43-118) with the reference's
-// Newton iteration: stop when the L1 (IE) / L2 (IM) residual norm is below
-// _epsilon1_, at most SODECL_NEWTON_MAXIT updates.  Snapshot j (state after
-// j * _numsteps_ steps) goes to out[j * neq ...].  Returns the Newton
-// updates taken.
-__device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const double* __restrict__ prm,
-                                        double* __restrict__ out, int steps, double* lds)
+// Integrates the orbits of one work item -- SODECL_OPW orbits, item[k] on
+// the k-th group of SODECL_LPO lanes (item[k] < 0: none) -- for steps[o]
+// steps of implicit Euler (SODECL_SOLVER 3, ie.cl:50-117) or implicit
+// midpoint (4, im.cl:43-118) with the reference's Newton iteration: stop
+// when the L1 (IE) / L2 (IM) residual norm is below _epsilon1_, at most
+// SODECL_NEWTON_MAXIT updates.  Snapshot j (state after j * _numsteps_
+// steps) goes to out[ooff[o] + j * neq ...]; newton[o] receives the updates
+// taken.  The orbits of a wavefront run in lockstep: the wavefront steps
+// until its longest orbit is done and iterates until every orbit's Newton
+// iteration has stopped; an orbit that has stopped computes along on its
+// last values without storing them or counting updates.
+__device__ void sodecl_batch_item(const int* __restrict__ item, const int* __restrict__ steps_of,
+                                  const long long* __restrict__ yoff, const long long* __restrict__ ooff,
+                                  const double* __restrict__ y0_all, const double* __restrict__ params_all,
+                                  double* __restrict__ out_all, long long* __restrict__ newton, double* lds)
 {
-    const int lane = threadIdx.x;
-    double* const sh_g = lds;                          // evaluation point (g, or (y+g)/2), plain layout
-    double* const sh_r = lds + _numeq_;                // residual -> right-hand side -> solution (padded layout)
+    const int part = threadIdx.x / SODECL_LPO;
+    const int lane = threadIdx.x % SODECL_LPO;
+    const int o = item[part];
+    const bool has = o >= 0;
+    const int steps = has ? steps_of[o] : 0;
+    int smax = steps;
+#if SODECL_OPW == 2
+    smax = max(__builtin_amdgcn_readlane(steps, 0), __builtin_amdgcn_readlane(steps, SODECL_LPO));
+#endif
+    double* const sh_g = lds + (size_t)part * SODECL_ORBIT_LDS;   // evaluation point (g, or (y+g)/2), plain layout
+    double* const sh_r = sh_g + _numeq_;           // residual -> right-hand side -> solution (padded layout)
     double* const sh_lo = sh_r + SODECL_TLEN;
     double* const sh_di = sh_lo + SODECL_TLEN;
     double* const sh_up = sh_di + SODECL_TLEN;
 #if _numpar_ <= 8
     // few parameters: a register copy, which the plugin's reads need not
     // reload from LDS after every store of the band rows
-    double sh_p[_numpar_];
+    double sh_p[_numpar_ > 0 ? _numpar_ : 1];
 #pragma unroll
-    for (int k = 0; k < _numpar_; ++k) sh_p[k] = prm[k];
+    for (int k = 0; k < _numpar_; ++k) sh_p[k] = has ? params_all[(size_t)o * _numpar_ + k] : 0.0;
 #else
     double* const sh_p = sh_up + SODECL_TLEN;
-    for (int k = lane; k < _numpar_; k += SODECL_BG) sh_p[k] = prm[k];
+    for (int k = lane; k < _numpar_; k += SODECL_LPO) sh_p[k] = has ? params_all[(size_t)o * _numpar_ + k] : 0.0;
 #endif
+    double* const out = has ? out_all + ooff[o] : nullptr;
     double y[SODECL_BR], gg[SODECL_BR];
 #pragma unroll
     for (int c = 0; c < SODECL_BR; ++c) {
-        const int comp = lane + c * SODECL_BG;
-        y[c] = SODECL_OWNS(comp) ? y0[comp] : 0.0;
-        if (SODECL_OWNS(comp)) out[comp] = y[c];
+        const int comp = lane + c * SODECL_LPO;
+        y[c] = 0.0;
+        if (has && SODECL_OWNS(comp)) {
+            y[c] = y0_all[yoff[o] + comp];
+            out[comp] = y[c];
+        }
     }
     double t = 0.0;
     long long updates = 0;
@@ -237,14 +283,15 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #else
     const double scale = -0.50;
 #endif
-    for (int s = 0; s < steps; ++s) {
+    for (int s = 0; s < smax; ++s) {
+        const bool live = s < steps;
 #pragma unroll
         for (int c = 0; c < SODECL_BR; ++c) gg[c] = y[c];
         for (int it = 0; it < SODECL_NEWTON_MAXIT; ++it) {
             __syncthreads();   // previous readers of sh_g are done
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
-                const int comp = lane + c * SODECL_BG;
+                const int comp = lane + c * SODECL_LPO;
 #if SODECL_SOLVER == 3
                 if (SODECL_OWNS(comp)) sh_g[comp] = gg[c];
 #else
@@ -256,7 +303,7 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
             double err = 0.0;
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
-                const int comp = lane + c * SODECL_BG;
+                const int comp = lane + c * SODECL_LPO;
                 r[c] = 0.0;
                 if (SODECL_OWNS(comp)) {
                     const double fv = sode_system_i(comp, t, sh_g, sh_p);
@@ -268,15 +315,16 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
 #endif
                 }
             }
-            err = sodecl_group_sum(err);
+            err = sodecl_orbit_sum(err);
 #if SODECL_SOLVER == 3
-            if (err < _epsilon1_) break;
+            const bool done = !live || err < _epsilon1_;
 #else
-            if (sqrt(err) < _epsilon1_) break;
+            const bool done = !live || sqrt(err) < _epsilon1_;
 #endif
+            if (__all(done)) break;
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
-                const int comp = lane + c * SODECL_BG;
+                const int comp = lane + c * SODECL_LPO;
                 if (SODECL_OWNS(comp)) {
                     double lo, di, up;
                     calc_jacobian_band_i(comp, sh_g, sh_p, &lo, &di, &up);
@@ -289,29 +337,36 @@ __device__ long long sodecl_batch_orbit(const double* __restrict__ y0, const dou
             __syncthreads();
             sodecl_tri_solve(sh_lo, sh_di, sh_up, sh_r, sh_r, lane);
             __syncthreads();
+            if (!done) {
 #pragma unroll
-            for (int c = 0; c < SODECL_BR; ++c) {
-                const int comp = lane + c * SODECL_BG;
-                if (SODECL_OWNS(comp)) gg[c] = gg[c] - sh_r[SODECL_TI(comp)];
+                for (int c = 0; c < SODECL_BR; ++c) {
+                    const int comp = lane + c * SODECL_LPO;
+                    if (SODECL_OWNS(comp)) gg[c] = gg[c] - sh_r[SODECL_TI(comp)];
+                }
+                ++updates;
             }
-            ++updates;
         }
+        if (live) {
 #pragma unroll
-        for (int c = 0; c < SODECL_BR; ++c) y[c] = gg[c];
+            for (int c = 0; c < SODECL_BR; ++c) y[c] = gg[c];
+        }
         t = t + _m_dt_;
-        if ((s + 1) % _numsteps_ == 0) {
-            double* o = out + (size_t)((s + 1) / _numsteps_) * _numeq_;
+        if (live && (s + 1) % _numsteps_ == 0) {
+            double* o_snap = out + (size_t)((s + 1) / _numsteps_) * _numeq_;
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
-                const int comp = lane + c * SODECL_BG;
-                if (SODECL_OWNS(comp)) o[comp] = y[c];
+                const int comp = lane + c * SODECL_LPO;
+                if (SODECL_OWNS(comp)) o_snap[comp] = y[c];
             }
         }
     }
-    return updates;
+    if (has && lane == 0) newton[o] = updates;
 }
 
 #undef SODECL_BR
+#undef SODECL_OPW
+#undef SODECL_PLDS
+#undef SODECL_ORBIT_LDS
 #undef SODECL_TI
 #undef SODECL_TLEN
 #undef SODECL_NPART

@@ -1,18 +1,21 @@
 // Persistent batched integrator: orbits of different system sizes and step
-// counts in one launch.  Each wavefront takes the next orbit from a device
-// work queue (an atomic index into a longest-first order the host builds
-// from neq * steps) and integrates it to the end with the routine of its
-// size class (batch_class.hip, one instance per class); every wave exits
-// once the queue is drained.
+// counts in one launch.  Each wavefront takes the next work item from a
+// device work queue (an atomic index into a longest-first item list the
+// host builds from neq * steps) and integrates its orbits to the end with
+// the routine of their size class (batch_class.hip, one instance per
+// class); every wave exits once the queue is drained.  An item holds one
+// orbit, or two orbits of the same class when that class runs 32 lanes per
+// orbit.
 //
-// Packed layouts: the initial state of orbit o at y0[yoff[o] .. + neq_o];
-// its snapshots [snap][neq_o] at out[ooff[o] ...], steps[o] / ksteps + 1 of
-// them; params[orbits][npar]; newton[o] receives the Newton updates taken.
+// Packed layouts: items[2 * item + k] = orbit or -1; the initial state of
+// orbit o at y0[yoff[o] .. + neq_o]; its snapshots [snap][neq_o] at
+// out[ooff[o] ...], steps[o] / ksteps + 1 of them; params[orbits][npar];
+// newton[o] receives the Newton updates taken.
 
 // At least two wavefronts per SIMD (a register budget of 256 VGPRs) for these
 // latency-bound wavefronts.
 extern "C" __global__ void __launch_bounds__(SODECL_BG) __attribute__((amdgpu_waves_per_eu(2)))
-sodecl_batch(const int* __restrict__ order, const int* __restrict__ cls, const int* __restrict__ steps,
+sodecl_batch(const int* __restrict__ items, const int* __restrict__ cls, const int* __restrict__ steps,
              const long long* __restrict__ yoff, const long long* __restrict__ ooff, const double* __restrict__ y0,
              const double* __restrict__ params, double* __restrict__ out, long long* __restrict__ newton,
              int* __restrict__ head, int nitems)
@@ -23,18 +26,13 @@ sodecl_batch(const int* __restrict__ order, const int* __restrict__ cls, const i
         if (threadIdx.x == 0) item = atomicAdd(head, 1);
         item = __shfl(item, 0, SODECL_BG);
         if (item >= nitems) break;
-        const int o = order[item];
-        const double* yo = y0 + yoff[o];
-        const double* po = params + (size_t)o * _numpar_;
-        double* oo = out + ooff[o];
-        long long nu = 0;
-        switch (cls[o]) {
+        const int* const io = items + (size_t)item * 2;
+        switch (cls[io[0]]) {
             SODECL_BATCH_CASES
         default:
             break;
         }
-        if (threadIdx.x == 0) newton[o] = nu;
-        __syncthreads();   // the next orbit reuses the LDS rows
+        __syncthreads();   // the next item reuses the LDS rows
     }
 }
 

@@ -195,16 +195,18 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
     for (size_t c = 0; c < class_neq.size(); ++c) {
         o << "namespace sodecl_cls" << c << " {\n";
         o << "#define _numeq_ " << class_neq[c] << "\n";
+        o << "#define SODECL_LPO " << batch_lanes_per_orbit(class_neq[c]) << "\n";
         o << "#pragma clang force_cuda_host_device begin\n";
         o << "#line 1 \"system.cl\"\n" << system_source << "\n";
         o << "#pragma clang force_cuda_host_device end\n";
         o << kBatchClassSource << "\n";
-        o << "#undef _numeq_\n";
+        o << "#undef _numeq_\n#undef SODECL_LPO\n";
         o << "}  // namespace sodecl_cls" << c << "\n";
     }
     o << "#define SODECL_BATCH_CASES";
     for (size_t c = 0; c < class_neq.size(); ++c)
-        o << " case " << c << ": nu = sodecl_cls" << c << "::sodecl_batch_orbit(yo, po, oo, steps[o], sodecl_lds); break;";
+        o << " case " << c << ": sodecl_cls" << c
+          << "::sodecl_batch_item(io, steps, yoff, ooff, y0, params, out, newton, sodecl_lds); break;";
     o << "\n";
     o << kBatchKernelSource << "\n";
     return o.str();

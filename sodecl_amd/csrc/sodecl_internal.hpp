@@ -75,6 +75,10 @@ struct Program {
 bool compile_program(const std::string& text, const std::string& arch, int fp_contract, std::vector<char>& code,
                      std::string& log, bool ilp_schedule = false);
 std::string device_arch(int device);
+// Lanes of a wavefront per orbit in the batched integrator for a size class
+// of neq equations: 32 (two orbits per wavefront) up to 128 equations, else
+// 64.
+inline int batch_lanes_per_orbit(int neq) { return neq <= 128 ? 32 : 64; }
 
 long num_launches(double tspan, double dt, int ksteps);
 
