@@ -24,7 +24,7 @@ struct sodecl_batch {
     int device = 0;
     // per-orbit layout
     int orbits = 0;
-    std::vector<int> cls, steps, order, items;   // items: pairs of orbits (-1: none), longest first
+    std::vector<int> cls, steps, order, items;   // items: four orbit slots each (-1: none), longest first
     int nitems = 0;
     std::vector<long long> yoff, ooff;
     size_t state_size = 0, output_size = 0;
@@ -179,29 +179,30 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     b->order.resize(orbits);
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
-    // work items: one orbit, or two consecutive orbits of the same class
-    // (similar costs) where that class runs 32 lanes per orbit
+    // work items: one orbit, or up to four consecutive orbits of the same
+    // class (similar costs) where that class runs 32 or 16 lanes per orbit
     const int nclass = (int)b->class_neq.size();
     std::vector<std::vector<int>> per_class(nclass);
     for (int o : b->order) per_class[b->cls[o]].push_back(o);
     struct Item {
         double cost;
-        int o0, o1;
+        int o[4];
     };
     std::vector<Item> items;
     for (int c = 0; c < nclass; ++c) {
         const int opw = 64 / batch_lanes_per_orbit(b->class_neq[c]);
         const auto& v = per_class[c];
-        for (size_t i = 0; i < v.size(); i += opw)
-            items.push_back({cost[v[i]], v[i], (opw == 2 && i + 1 < v.size()) ? v[i + 1] : -1});
+        for (size_t i = 0; i < v.size(); i += opw) {
+            Item it{cost[v[i]], {-1, -1, -1, -1}};
+            for (int k = 0; k < opw && i + k < v.size(); ++k) it.o[k] = v[i + k];
+            items.push_back(it);
+        }
     }
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) { return a.cost > c.cost; });
     b->nitems = (int)items.size();
-    b->items.resize(2 * items.size());
-    for (size_t i = 0; i < items.size(); ++i) {
-        b->items[2 * i] = items[i].o0;
-        b->items[2 * i + 1] = items[i].o1;
-    }
+    b->items.resize(4 * items.size());
+    for (size_t i = 0; i < items.size(); ++i)
+        for (int k = 0; k < 4; ++k) b->items[4 * i + k] = items[i].o[k];
 
     if (!bhip(b, hipSetDevice(b->device), "hipSetDevice")) return 0;
     b->program = assemble_batch_program(b->source, batch_key(b), b->features, b->class_neq);

@@ -73,12 +73,22 @@ SODECL_INLINE void sodecl_pcr_pair32(double v, int lane, double& up, double& dow
 }
 
 // v of orbit lane - s (up) or lane + s (down), 0 past either end of the
-// orbit's lanes; distances of one and two are DPP wavefront shifts (with a
-// whole wavefront per orbit the shift itself brings 0 past either end;
-// narrower orbits drop what crossed over from the neighbouring orbit),
-// longer ones a bpermute.
+// orbit's lanes.  Orbits of 16 lanes are DPP rows: row shifts, which bring
+// 0 past either end by themselves.  Otherwise distances of one and two are
+// DPP wavefront shifts (with a whole wavefront per orbit the shift brings 0
+// past either end; 32-lane orbits drop what crossed over from the
+// neighbouring orbit), longer ones a bpermute.
 SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
 {
+#if SODECL_LPO == 16
+    (void)lane;
+    switch (s) {
+    case 1: return up ? sodecl_dpp<SODECL_DPP_ROW_SHR(1)>(v) : sodecl_dpp<SODECL_DPP_ROW_SHL(1)>(v);
+    case 2: return up ? sodecl_dpp<SODECL_DPP_ROW_SHR(2)>(v) : sodecl_dpp<SODECL_DPP_ROW_SHL(2)>(v);
+    case 4: return up ? sodecl_dpp<SODECL_DPP_ROW_SHR(4)>(v) : sodecl_dpp<SODECL_DPP_ROW_SHL(4)>(v);
+    default: return up ? sodecl_dpp<SODECL_DPP_ROW_SHR(8)>(v) : sodecl_dpp<SODECL_DPP_ROW_SHL(8)>(v);
+    }
+#endif
     const bool inside = up ? lane - s >= 0 : lane + s < SODECL_LPO;
     if (s <= 2) {
         double w = up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v) : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v);
@@ -99,7 +109,9 @@ SODECL_INLINE double sodecl_orbit_sum(double v)
     v += sodecl_dpp<SODECL_DPP_XOR2>(v);
     v += sodecl_dpp<SODECL_DPP_HALF_MIRROR>(v);
     v += sodecl_dpp<SODECL_DPP_MIRROR>(v);
+#if SODECL_LPO == 32
     v += __shfl_xor(v, 16, SODECL_LPO);
+#endif
     return v;
 #endif
 }
@@ -225,8 +237,8 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     }
 }
 
-// Integrates the orbits of one work item -- SODECL_OPW orbits, item[k] on
-// the k-th group of SODECL_LPO lanes (item[k] < 0: none) -- for steps[o]
+// Integrates the orbits of one work item -- SODECL_OPW orbits (1, 2 or 4),
+// item[k] on the k-th group of SODECL_LPO lanes (item[k] < 0: none) -- for steps[o]
 // steps of implicit Euler (SODECL_SOLVER 3, ie.cl:50-117) or implicit
 // midpoint (4, im.cl:43-118) with the reference's Newton iteration: stop
 // when the L1 (IE) / L2 (IM) residual norm is below _epsilon1_, at most
@@ -247,8 +259,11 @@ __device__ void sodecl_batch_item(const int* __restrict__ item, const int* __res
     const bool has = o >= 0;
     const int steps = has ? steps_of[o] : 0;
     int smax = steps;
-#if SODECL_OPW == 2
+#if SODECL_OPW >= 2
     smax = max(__builtin_amdgcn_readlane(steps, 0), __builtin_amdgcn_readlane(steps, SODECL_LPO));
+#endif
+#if SODECL_OPW == 4
+    smax = max(smax, max(__builtin_amdgcn_readlane(steps, 2 * SODECL_LPO), __builtin_amdgcn_readlane(steps, 3 * SODECL_LPO)));
 #endif
     double* const sh_g = lds + (size_t)part * SODECL_ORBIT_LDS;   // evaluation point (g, or (y+g)/2), plain layout
     double* const sh_r = sh_g + _numeq_;           // residual -> right-hand side -> solution (padded layout)

@@ -4,10 +4,10 @@
 // host builds from neq * steps) and integrates its orbits to the end with
 // the routine of their size class (batch_class.hip, one instance per
 // class); every wave exits once the queue is drained.  An item holds one
-// orbit, or two orbits of the same class when that class runs 32 lanes per
-// orbit.
+// orbit, or up to 64 / lanes-per-orbit orbits of the same class when that
+// class runs 32 or 16 lanes per orbit.
 //
-// Packed layouts: items[2 * item + k] = orbit or -1; the initial state of
+// Packed layouts: items[4 * item + k] = orbit or -1; the initial state of
 // orbit o at y0[yoff[o] .. + neq_o]; its snapshots [snap][neq_o] at
 // out[ooff[o] ...], steps[o] / ksteps + 1 of them; params[orbits][npar];
 // newton[o] receives the Newton updates taken.
@@ -26,7 +26,7 @@ sodecl_batch(const int* __restrict__ items, const int* __restrict__ cls, const i
         if (threadIdx.x == 0) item = atomicAdd(head, 1);
         item = __shfl(item, 0, SODECL_BG);
         if (item >= nitems) break;
-        const int* const io = items + (size_t)item * 2;
+        const int* const io = items + (size_t)item * 4;
         switch (cls[io[0]]) {
             SODECL_BATCH_CASES
         default:

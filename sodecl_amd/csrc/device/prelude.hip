@@ -17,6 +17,8 @@ __device__ inline double sodecl_dpp(double v)
 #define SODECL_DPP_MIRROR 0x140       // lane i <- 15 - i within each 16 lanes
 #define SODECL_DPP_WAVE_SHR1 0x138    // lane i <- lane i - 1 across the wavefront (lane 0 <- 0)
 #define SODECL_DPP_WAVE_SHL1 0x130    // lane i <- lane i + 1 across the wavefront (lane 63 <- 0)
+#define SODECL_DPP_ROW_SHR(n) (0x110 + (n))   // lane i <- lane i - n within its 16-lane row (0 past the start)
+#define SODECL_DPP_ROW_SHL(n) (0x100 + (n))   // lane i <- lane i + n within its 16-lane row (0 past the end)
 
 // Under the group schedule SODECL_GROUP consecutive lanes of a wavefront
 // share one orbit.  A plugin entry point that the whole group calls together

@@ -76,9 +76,9 @@ bool compile_program(const std::string& text, const std::string& arch, int fp_co
                      std::string& log, bool ilp_schedule = false);
 std::string device_arch(int device);
 // Lanes of a wavefront per orbit in the batched integrator for a size class
-// of neq equations: 32 (two orbits per wavefront) up to 128 equations, else
-// 64.
-inline int batch_lanes_per_orbit(int neq) { return neq <= 128 ? 32 : 64; }
+// of neq equations: 16 (four orbits per wavefront) up to 32 equations, 32
+// (two) up to 128, else 64.
+inline int batch_lanes_per_orbit(int neq) { return neq <= 32 ? 16 : neq <= 128 ? 32 : 64; }
 
 long num_launches(double tspan, double dt, int ksteps);
 
