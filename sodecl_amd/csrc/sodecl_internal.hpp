@@ -78,7 +78,7 @@ std::string device_arch(int device);
 // Lanes of a wavefront per orbit in the batched integrator for a size class
 // of neq equations: 16 (four orbits per wavefront) up to 32 equations, 32
 // (two) up to 128, else 64.
-inline int batch_lanes_per_orbit(int neq) { return neq <= 32 ? 16 : neq <= 128 ? 32 : 64; }
+inline int batch_lanes_per_orbit(int neq) { return neq <= 128 ? 16 : 64; }
 
 long num_launches(double tspan, double dt, int ksteps);
 
