@@ -237,6 +237,13 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     }
 }
 
+// With several size classes in one kernel each class's routine is a
+// function of its own (not inlined), so that registers are allocated per
+// class rather than across all of them (measured: no spills instead of 55
+// spilled VGPRs for 32/128/512-equation classes, 2-3% faster).
+#ifndef SODECL_BATCH_ITEM_ATTR
+#define SODECL_BATCH_ITEM_ATTR
+#endif
 // Integrates the orbits of one work item -- SODECL_OPW orbits (1, 2 or 4),
 // item[k] on the k-th group of SODECL_LPO lanes (item[k] < 0: none) -- for steps[o]
 // steps of implicit Euler (SODECL_SOLVER 3, ie.cl:50-117) or implicit
@@ -248,7 +255,7 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
 // until its longest orbit is done and iterates until every orbit's Newton
 // iteration has stopped; an orbit that has stopped computes along on its
 // last values without storing them or counting updates.
-__device__ void sodecl_batch_item(const int* __restrict__ item, const int* __restrict__ steps_of,
+__device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__ item, const int* __restrict__ steps_of,
                                   const long long* __restrict__ yoff, const long long* __restrict__ ooff,
                                   const double* __restrict__ y0_all, const double* __restrict__ params_all,
                                   double* __restrict__ out_all, long long* __restrict__ newton, double* lds)

@@ -191,6 +191,7 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
     o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
     o << kPreludeSource << "\n";
     (void)f;
+    if (class_neq.size() > 1) o << "#define SODECL_BATCH_ITEM_ATTR __attribute__((noinline))\n";
     // one namespace per size class: the plugin and the per-class solver
     for (size_t c = 0; c < class_neq.size(); ++c) {
         o << "namespace sodecl_cls" << c << " {\n";
