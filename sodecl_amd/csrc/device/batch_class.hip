@@ -88,7 +88,7 @@ SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
     case 4: return up ? sodecl_dpp<SODECL_DPP_ROW_SHR(4)>(v) : sodecl_dpp<SODECL_DPP_ROW_SHL(4)>(v);
     default: return up ? sodecl_dpp<SODECL_DPP_ROW_SHR(8)>(v) : sodecl_dpp<SODECL_DPP_ROW_SHL(8)>(v);
     }
-#endif
+#else
     const bool inside = up ? lane - s >= 0 : lane + s < SODECL_LPO;
     if (s <= 2) {
         double w = up ? sodecl_dpp<SODECL_DPP_WAVE_SHR1>(v) : sodecl_dpp<SODECL_DPP_WAVE_SHL1>(v);
@@ -97,6 +97,7 @@ SODECL_INLINE double sodecl_pcr_neighbour(double v, int lane, int s, bool up)
     }
     const double w = up ? __shfl_up(v, (unsigned)s, SODECL_LPO) : __shfl_down(v, (unsigned)s, SODECL_LPO);
     return inside ? w : 0.0;
+#endif
 }
 
 // Sum over the orbit's lanes, the same value on each of them.
