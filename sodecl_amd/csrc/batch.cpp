@@ -222,7 +222,8 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
         const long lpo = batch_lanes_per_orbit((int)n);
         const long br = (n + lpo - 1) / lpo;
         const long tlen = br >= 2 ? n + lpo : n;
-        const long per_orbit = n + 4 * tlen + (b->npar <= 8 ? 0 : b->npar);
+        long per_orbit = n + 4 * tlen + (b->npar <= 8 ? 0 : b->npar);
+        if (lpo < 64) per_orbit = (per_orbit + 31) / 32 * 32 + 16;   // bank offset between orbits
         lds_doubles = std::max(lds_doubles, (size_t)(per_orbit * (64 / lpo)));
     }
     b->lds_bytes = lds_doubles * sizeof(double);

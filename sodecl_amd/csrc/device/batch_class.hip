@@ -38,7 +38,11 @@
 // LDS doubles per orbit: evaluation point, four padded solve rows, and the
 // parameters when there are more than 8 (fewer stay in registers)
 #define SODECL_PLDS (_numpar_ <= 8 ? 0 : _numpar_)
-#define SODECL_ORBIT_LDS (_numeq_ + 4 * SODECL_TLEN + SODECL_PLDS)
+// Orbits sharing a wavefront start 16 doubles (32 banks) apart modulo 32
+// doubles, so the same access by neighbouring orbits falls on disjoint LDS
+// banks.
+#define SODECL_ORBIT_LDS_MIN (_numeq_ + 4 * SODECL_TLEN + SODECL_PLDS)
+#define SODECL_ORBIT_LDS (SODECL_OPW > 1 ? (SODECL_ORBIT_LDS_MIN + 31) / 32 * 32 + 16 : SODECL_ORBIT_LDS_MIN)
 
 // Coefficients of row i of the reference's tridiagonal matrix.
 SODECL_INLINE double sodecl_tri_a(const double* up, int i)
@@ -390,6 +394,7 @@ __device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__
 #undef SODECL_OPW
 #undef SODECL_PLDS
 #undef SODECL_ORBIT_LDS
+#undef SODECL_ORBIT_LDS_MIN
 #undef SODECL_TI
 #undef SODECL_TLEN
 #undef SODECL_NPART
