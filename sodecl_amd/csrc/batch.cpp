@@ -221,9 +221,8 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
         const long n = b->class_neq[c];
         const long lpo = batch_lanes_per_orbit((int)n);
         const long br = (n + lpo - 1) / lpo;
-        const long tlen = br >= 2 ? n + lpo : n;
-        long per_orbit = n + 4 * tlen + (b->npar <= 8 ? 0 : b->npar);
-        if (lpo < 64) per_orbit = (per_orbit + 31) / 32 * 32 + 16;   // bank offset between orbits
+        const long tlen = br >= 2 ? (n + 15) / 16 * 16 : n;
+        const long per_orbit = n + 4 * tlen + (b->npar <= 8 ? 0 : b->npar);
         lds_doubles = std::max(lds_doubles, (size_t)(per_orbit * (64 / lpo)));
     }
     b->lds_bytes = lds_doubles * sizeof(double);
