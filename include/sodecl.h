@@ -81,9 +81,10 @@ typedef struct sodecl_options {
     int verbose;             /* print "Compute results runtime" like the reference */
     size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
     int pairwise;            /* group schedule with a plugin that defines the pairwise drift (sode_pair,
-                                sode_system_pairsum): 0 = automatic (lane exchange when the group has a
-                                lane per component, else the full N x (N+1) LDS matrix while it is at
-                                most 16 KiB per orbit, else off), 1 = always (lane exchange, else half
+                                sode_system_pairsum): 0 = automatic (the full N x (N+1) LDS matrix when
+                                the group has two lanes per component and the matrix is at most 16 KiB
+                                per orbit, else lane exchange when the group has a lane per component,
+                                else the full matrix while it fits, else off), 1 = always (lane exchange, else half
                                 storage), 2 = half storage, 3 = full matrix, 4 = lane exchange only,
                                 -1 = never.  The LDS layouts reproduce the per-component drift bit for
                                 bit; the lane exchange sums the same terms in another order */
