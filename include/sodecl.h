@@ -77,7 +77,9 @@ typedef struct sodecl_options {
                                 launch sequence once as a hipGraph and replay it */
     int autotune;            /* with schedule AUTO: time the candidate schedules, group sizes and block
                                 sizes on one segment at setup and keep the fastest (cached on disk) */
-    int fp_contract;         /* 0 = fused multiply-add contraction allowed (default), 1 = off */
+    int fp_contract;         /* 0 = fused multiply-add contraction of the system source's and the solvers'
+                                operations allowed (default; the device library's functions are compiled
+                                as written), 1 = off */
     int verbose;             /* print "Compute results runtime" like the reference */
     size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
     int pairwise;            /* group schedule with a plugin that defines the pairwise drift (sode_pair,
@@ -90,6 +92,11 @@ typedef struct sodecl_options {
                                 bit; the lane exchange sums the same terms in another order */
     double newton_tol;       /* Newton stop threshold of the implicit solvers (_epsilon1_); 0 = 1e-07
                                 (src/sodeclmgr.cpp:424-430) */
+    int library_sin;         /* sin() of a double in the system source: 0 = SODECL's own sine (default;
+                                Cody-Waite reduction by pi and a series, at most 0.87 ulp measured,
+                                about 40% of the device library's instructions; the library's sine
+                                for |x| > 2^30 and within ~|x| 1.6e-16 of a nonzero multiple of pi),
+                                1 = the device library's sin */
 } sodecl_options;
 
 typedef struct sodecl_ctx sodecl_ctx;

@@ -80,6 +80,7 @@ ProgramKey batch_key(const sodecl_batch* b)
     k.newton_max_iter = b->opts.newton_max_iter > 0 ? b->opts.newton_max_iter : 1000;
     k.newton_tol = b->opts.newton_tol;
     k.fp_contract = b->opts.fp_contract;
+    k.library_sin = b->opts.library_sin;
     return k;
 }
 

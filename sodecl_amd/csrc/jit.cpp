@@ -115,6 +115,12 @@ std::string epsilon_define(const ProgramKey& k)
     return buf;
 }
 
+// sin() of the system source resolves to sodecl_sin (device/sin.hip) unless
+// the caller asked for the device library's; the solver templates after the
+// system source keep the library's functions.
+std::string system_sin_begin(const ProgramKey& k) { return k.library_sin ? "" : "#define sin sodecl_sin\n"; }
+std::string system_sin_end(const ProgramKey& k) { return k.library_sin ? "" : "#undef sin\n"; }
+
 std::string assemble_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f)
 {
     std::ostringstream o;
@@ -156,8 +162,10 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     o << kPreludeSource << "\n";
     // The plugin's functions become host+device functions.
     o << "#pragma clang force_cuda_host_device begin\n";
+    o << system_sin_begin(k);
     o << "#line 1 \"system.cl\"\n";
     o << system_source << "\n";
+    o << system_sin_end(k);
     o << "#pragma clang force_cuda_host_device end\n";
     o << "#line 1 \"integrators.hip\"\n";
     o << kIntegratorsSource << "\n";
@@ -198,7 +206,7 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
         o << "#define _numeq_ " << class_neq[c] << "\n";
         o << "#define SODECL_LPO " << batch_lanes_per_orbit(class_neq[c]) << "\n";
         o << "#pragma clang force_cuda_host_device begin\n";
-        o << "#line 1 \"system.cl\"\n" << system_source << "\n";
+        o << system_sin_begin(k) << "#line 1 \"system.cl\"\n" << system_source << "\n" << system_sin_end(k);
         o << "#pragma clang force_cuda_host_device end\n";
         o << kBatchClassSource << "\n";
         o << "#undef _numeq_\n#undef SODECL_LPO\n";
@@ -287,7 +295,11 @@ bool compile_program(const std::string& text, const std::string& arch, int fp_co
     int vmaj = 0, vmin = 0;
     hiprtcVersion(&vmaj, &vmin);
     std::vector<std::string> opts = {"--gpu-architecture=" + arch, "-O3", "-std=c++17",
-                                     fp_contract ? "-ffp-contract=off" : "-ffp-contract=fast",
+                                     // contraction marked on the program's own operations only:
+                                     // "fast" would also fuse products and sums inside the device
+                                     // library's functions, whose double-double steps need them
+                                     // separate (its sine of 1e22 then comes out 0.02 off)
+                                     fp_contract ? "-ffp-contract=off" : "-ffp-contract=fast-honor-pragmas",
                                      // no packed FP32 instructions in any generated code
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
     // One wavefront per SIMD leaves no other wavefront to hide latency

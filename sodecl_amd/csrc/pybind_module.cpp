@@ -46,6 +46,7 @@ void apply_options(sodecl_options& o, const py::dict& d)
         else if (k == "traj_budget_bytes") o.traj_budget_bytes = kv.second.cast<size_t>();
         else if (k == "pairwise") o.pairwise = kv.second.cast<int>();
         else if (k == "newton_tol") o.newton_tol = kv.second.cast<double>();
+        else if (k == "library_sin") o.library_sin = kv.second.cast<int>();
         else throw std::invalid_argument("unknown option " + k);
     }
 }

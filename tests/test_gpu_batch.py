@@ -68,6 +68,28 @@ def test_batch_odd_sizes(cells):
     check_against_oracle(cfg, out, nu, yoff, ooff, range(40))
 
 
+@pytest.mark.parametrize("cells", [(16,), (16, 256), (8, 64, 256)])
+def test_batch_items_with_unequal_lengths(cells):
+    """Orbits that share a work item (up to four small orbits run in lockstep
+    on one wavefront) but need very different step counts, zero included,
+    with orbit counts that leave items partly empty; the second run of the
+    same batch (work queue reset, no state kept) is bit-identical."""
+    orbits = 11
+    cfg = configs.brusselator_batch(orbits=orbits, seed=6, solver=4, cells=cells, smin=50, smax=100, ksteps=50,
+                                    newton_max_iter=1000)
+    cfg["steps"] = np.array([0, 1000, 50, 0, 600, 50, 950, 0, 100, 1000, 50], dtype=np.int32)[:orbits]
+    out, nu, yoff, ooff = run_batch(cfg)
+    zero = [k for k in range(orbits) if cfg["steps"][k] == 0]
+    for k in zero:
+        n = cfg["class_neq"][cfg["cls"][k]]
+        np.testing.assert_array_equal(out[ooff[k]:ooff[k] + n], cfg["y0"][yoff[k]:yoff[k] + n])
+        assert nu[k] == 0
+    assert check_against_oracle(cfg, out, nu, yoff, ooff, [k for k in range(orbits) if k not in zero]) <= 1
+    out2, nu2, _, _ = run_batch(cfg)
+    np.testing.assert_array_equal(out2, out)
+    np.testing.assert_array_equal(nu2, nu)
+
+
 def test_batch_reference_newton_cap():
     """With the reference's cap (1000) the Newton iteration converges and the
     update counts match the oracle's."""
