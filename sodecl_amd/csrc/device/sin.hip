@@ -5,29 +5,29 @@
 //
 // Method: Cody and Waite's additive range reduction (W. J. Cody, W. Waite,
 // "Software Manual for the Elementary Functions", 1980) by pi, with the
-// rounding error of the reduction carried as a tail, and the Taylor series
-// of sin:
-//   k = rint(x / pi), r = x - k pi in [-pi/2, pi/2], sin x = (-1)^k sin r.
-//   pi ~ P1 + P2 + P3: P1 is pi rounded to double, P2 the remainder rounded,
-//   P3 the next remainder rounded (|pi - P1 - P2 - P3| < 1.2e-49).
-//   r1 = x - k P1 is exact: for |x| >= 2 both x and k P1 are multiples of
-//   2^-51 and |r1| < 2; for |x| < 2, k is 0 or +-1 and both are multiples of
-//   2^-52.  r2 = round(r1 - k P2); while |r1| >= 2 |k P2|, r1 - r2 is exact
-//   (Sterbenz) and t = (r1 - r2 - k P2) - k P3 is r2's rounding error less
-//   the third term, so r = r2 + t to about 2^-100 relative.
-//   sin(r2 + t) = r2 + S1 r2^3 + r2^5 (S2 + z S3 + ... + z^8 S10)
-//                 + t (1 - z/2 + z^2/24),   z = r2^2, S_n = (-1)^n / (2n+1)!
-//   The first omitted series term is below 1.2e-18 on |r| <= pi/2 and the
-//   tail's below 0.02 ulp.  The leading correction S1 r2^3 (up to 0.65 in
-//   size against a result near 1) is formed without error: r2^3 = v + vl from
-//   exact products, S1 = -1/6 as a sum of two doubles, and u = r2 + v S1
-//   with its rounding error e (r2 - u is exact since |v S1| <= |r2| / 2).
-//   Measured on 8e6 arguments: at most 0.87 ulp from the exact sine.
+// rounding error of the reduction carried as a tail, and a polynomial for
+// the series remainder fitted by Chebyshev interpolation (near-minimax):
+//   sin x = sign(x) (-1)^k sin r,  k = round(|x| / pi),  r = |x| - k pi.
+//   pi ~ P1 + P2: P1 is pi rounded to double, P2 the remainder rounded
+//   (|pi - P1 - P2| < 3.1e-33).  r1 = |x| - k P1 is exact: for |x| >= 2 both
+//   terms are multiples of 2^-51 and |r1| < 2; for |x| < 2, k is 0 or 1 and
+//   both are multiples of 2^-52.  r2 = round(r1 - k P2); while
+//   |r1| >= 2 k P2, r1 - r2 is exact (Sterbenz) and t = r1 - r2 - k P2 is
+//   r2's rounding error, so r = r2 + t up to k |pi - P1 - P2|, which is
+//   below 2^-56 |r| whenever |r2| >= 4 k P2.
+//   sin(r2 + t) = r2 - r2^3 / 6 + r2^5 P(r2^2) + t (1 - z/2 + z^2/24),
+//   z = r2^2, P of degree 7 in z, within 7.7e-21 of (sin r - r + r^3/6) / r^5
+//   on |r| <= pi/2 (mpmath chebyfit, 60 digits); the tail's omitted terms
+//   are below 0.02 ulp.  The leading correction -r2^3 / 6 (up to 0.65 in size
+//   against a result near 1) is formed without error: r2^3 = v + vl from
+//   exact products, -1/6 as a sum of two doubles, and u = r2 + v (-1/6) with
+//   its rounding error e (r2 - u is exact since |v| / 6 <= |r2| / 2).
+//   Measured on 2e7 arguments: at most 0.88 ulp from the exact sine.
 // The device library's sin(x) is used instead where this does not hold:
-// |x| > 2^30 (and NaN or infinite x), and x within 4 |k P2| (about
-// |k| 4.9e-16) of a nonzero multiple of pi.
+// |x| > 2^30 (and NaN or infinite x), and x within 4 k P2 (about
+// k 4.9e-16) of a nonzero multiple k pi.
 //
-// Cost on gfx950: about 38 VALU instructions, against about 64 for the
+// Cost on gfx950: about 35 VALU instructions, against about 54 for the
 // device library's sine (which reduces by pi/2 in double-double and evaluates
 // both the sine and cosine polynomials).
 
@@ -57,26 +57,25 @@ SODECL_MATH_FN double sodecl_with_hi_word(double v, unsigned hi)
 // change the result.
 SODECL_MATH_FN double sodecl_sin_reduced(double x, bool* library)
 {
-    const double inv_pi = 0x1.45f306dc9c883p-2;
+    const double ax = fabs(x);
     const double p1 = 0x1.921fb54442d18p+1;
     const double p2 = 0x1.1a62633145c07p-53;
-    const double p3 = -0x1.f1976b7ed8fbcp-109;
-    const double k = rint(x * inv_pi);
-    const double r1 = fma(-k, p1, x);
+    // kk = k + 2^52 with k = round(|x| / pi): k's parity is kk's lowest bit
+    const double kk = fma(ax, 0x1.45f306dc9c883p-2, 0x1p52);
+    const double k = kk - 0x1p52;
+    const double r1 = fma(-k, p1, ax);
     const double r2 = fma(-k, p2, r1);
-    double t = fma(-k, p2, r1 - r2);
-    t = fma(-k, p3, t);
-    *library = !(fabs(x) <= 0x1p30) || fabs(r2) < fabs(k) * (4.0 * p2);
+    const double t = fma(-k, p2, r1 - r2);
+    *library = !(ax <= 0x1p30) || fabs(r2) < k * (4.0 * p2);
     const double z = r2 * r2;
-    double p = 0x1.71b8ef6dcf572p-66;        //  1/21!
-    p = fma(p, z, -0x1.2f49b46814157p-57);   // -1/19!
-    p = fma(p, z, 0x1.952c77030ad4ap-49);    //  1/17!
-    p = fma(p, z, -0x1.ae7f3e733b81fp-41);   // -1/15!
-    p = fma(p, z, 0x1.6124613a86d09p-33);    //  1/13!
-    p = fma(p, z, -0x1.ae64567f544e4p-26);   // -1/11!
-    p = fma(p, z, 0x1.71de3a556c734p-19);    //  1/9!
-    p = fma(p, z, -0x1.a01a01a01a01ap-13);   // -1/7!
-    p = fma(p, z, 0x1.1111111111111p-7);     //  1/5!
+    double p = -0x1.283dfd82ab975p-57;
+    p = fma(p, z, 0x1.951051ba25afep-49);
+    p = fma(p, z, -0x1.ae7f03ca459a2p-41);
+    p = fma(p, z, 0x1.612460f6bebccp-33);
+    p = fma(p, z, -0x1.ae64567eff356p-26);
+    p = fma(p, z, 0x1.71de3a556c3ecp-19);
+    p = fma(p, z, -0x1.a01a01a01a018p-13);
+    p = fma(p, z, 0x1.1111111111111p-7);
     const double zl = fma(r2, r2, -z);       // r2^2 = z + zl
     const double v = z * r2;                 // r2^3 = v + vl
     const double vl = fma(zl, r2, fma(z, r2, -v));
@@ -88,10 +87,11 @@ SODECL_MATH_FN double sodecl_sin_reduced(double x, bool* library)
     w = fma(vl, s1, w);
     w = fma(t, fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0), w);   // + t (1 - z/2 + z^2/24)
     const double s = u + w;
-    // (-1)^k, and the sign of a zero argument (r2 is +0 for x = -0)
-    unsigned hi = sodecl_hi_word(s) ^ ((unsigned)(int)k << 31);
-    hi = x == 0.0 ? sodecl_hi_word(x) : hi;
-    return sodecl_with_hi_word(s, hi);
+    // sin x = sign(x) (-1)^k sin r
+    unsigned lo;
+    __builtin_memcpy(&lo, &kk, sizeof lo);
+    const unsigned flip = ((lo << 31) ^ sodecl_hi_word(x)) & 0x80000000u;
+    return sodecl_with_hi_word(s, sodecl_hi_word(s) ^ flip);
 }
 
 SODECL_MATH_FN double sodecl_sin(double x)
