@@ -487,7 +487,20 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
             if (src < 0) src += _numeq_;
         }
         const double v = row ? sode_pair(i, j, t, x, pp) : 0.0;
-        const double w = __shfl(v, src, SODECL_G);
+        double w;
+        if (_numeq_ == SODECL_G && (SODECL_G & (SODECL_G - 1)) == 0) {
+            // ds_bpermute selects the source lane by address bits 7..2, so
+            // (i - d) * 4 wraps around a whole wavefront by itself; a smaller
+            // group keeps the wrapped offset inside its own lanes
+            int addr = (i - d) * 4;
+            if (SODECL_G < 64) addr = (addr & (4 * SODECL_G - 1)) | (int)((threadIdx.x & (64 - SODECL_G)) * 4);
+            const long long bits = __double_as_longlong(v);
+            const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(bits & 0xffffffffll));
+            const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(bits >> 32));
+            w = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+        } else {
+            w = __shfl(v, src, SODECL_G);
+        }
         acc = acc + v;
         acc = acc - w;
     }
