@@ -17,7 +17,7 @@
 //   below 2^-56 |r| whenever |r2| >= 4 k P2.
 //   sin(r2 + t) = r2 - r2^3 / 6 + r2^5 P(r2^2) + t (1 - z/2 + z^2/24),
 //   z = r2^2, P of degree 7 in z, within 7.7e-21 of (sin r - r + r^3/6) / r^5
-//   on |r| <= pi/2 (mpmath chebyfit, 60 digits); the tail's omitted terms
+//   on |r| <= pi/2 (sin_coefficients.py derives every constant here); the tail's omitted terms
 //   are below 0.02 ulp.  The leading correction -r2^3 / 6 (up to 0.65 in size
 //   against a result near 1) is formed without error: r2^3 = v + vl from
 //   exact products, -1/6 as a sum of two doubles, and u = r2 + v (-1/6) with
