@@ -127,6 +127,22 @@ def test_kuramoto_group_vs_oracle(n, solver, group):
     assert_traj_close(lane, ref["out"], what="lane")
 
 
+@pytest.mark.parametrize("n,solver,group", [(10, 2, 64), (64, 2, 64), (32, 1, 32)])
+def test_kuramoto_with_library_sine_vs_oracle(n, solver, group):
+    """With library_sin = 1 the system source's sin() is the device library's:
+    the same parity bar, and results that differ from the default sine's only
+    by rounding."""
+    cfg = configs.kuramoto_euler(orbits=203, n=n, seed=n + 1)
+    cfg["solver"] = solver
+    cfg["tspan"] = 1.0
+    _, lib = run_session(cfg, schedule="group", group_size=group, library_sin=1)
+    _, own = run_session(cfg, schedule="group", group_size=group)
+    ref = oracle_run(cfg, "kuramoto")
+    assert_traj_close(lib, ref["out"], what=f"library sine N={n}")
+    assert_traj_close(own, ref["out"], what=f"own sine N={n}")
+    assert not np.array_equal(lib, own)
+
+
 @pytest.mark.parametrize("n,group,ksteps", [(3, 4, 20), (32, 32, 20), (32, 16, 20), (5, 8, 20), (5, 8, 7), (3, 64, 9)])
 def test_kuramoto_em_vs_oracle(n, group, ksteps):
     """Groups with at least twice as many lanes as Threefry calls draw two
