@@ -103,7 +103,7 @@ SODECL_MATH_FN double sodecl_sin_reduced(double x, bool* library)
     w = fma(t, fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0), w);   // + t (1 - z/2 + z^2/24)
     const double s = u + w;
     // sin x = sign(x) (-1)^k sin r
-    unsigned lo;
+    unsigned lo;   // low word of kk (little-endian on the host and the GPU): k mod 2^32
     __builtin_memcpy(&lo, &kk, sizeof lo);
     const unsigned flip = ((lo << 31) ^ sodecl_hi_word(x)) & 0x80000000u;
     return sodecl_with_hi_word(s, sodecl_hi_word(s) ^ flip);
