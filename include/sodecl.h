@@ -29,7 +29,8 @@
 extern "C" {
 #endif
 
-#define SODECL_AMD_ABI_VERSION 1
+/* 2: sodecl_options gained library_sin (callers compiled against 1 pass a shorter struct) */
+#define SODECL_AMD_ABI_VERSION 2
 
 /* include/solver_Type.hpp:14-20 */
 enum sodecl_solver {
