@@ -117,5 +117,8 @@ SODECL_MATH_FN double sodecl_sin(double x)
     return s;
 }
 
-// Single precision keeps the library's sinf.
+// Single precision keeps the library's sinf; integer arguments convert to
+// double, as the library's sin(int) does.
 SODECL_MATH_FN float sodecl_sin(float x) { return sinf(x); }
+template <typename T>
+SODECL_MATH_FN double sodecl_sin(T x) { return sodecl_sin((double)x); }

@@ -97,3 +97,18 @@ def test_library_sin_option_is_validated():
     s = si.Session(sodecl_amd.system_path("kuramoto"), 2, 64, 10, 11, 0, 0.01, 1.0, 10, 0, {})
     with pytest.raises(RuntimeError, match="library_sin"):
         s.set_options({"library_sin": 2})
+
+
+def test_sin_overloads_compile():
+    """sin() of a float keeps single precision, sin() of an integer converts
+    to double, as with the library's overloads."""
+    src = """
+void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
+{
+    float a = sin(0.5f);
+    int n = 2;
+    yout[0] = sin(y[0]) + (double)a + sin(n) + sin(1L);
+}
+"""
+    s = si.Session(src, 1, 64, 1, 1, 0, 0.1, 1.0, 1, 0, {}, True)
+    assert s.compile_only("gfx950") > 1000
