@@ -94,8 +94,8 @@ typedef struct sodecl_options {
     double newton_tol;       /* Newton stop threshold of the implicit solvers (_epsilon1_); 0 = 1e-07
                                 (src/sodeclmgr.cpp:424-430) */
     int library_sin;         /* sin() of a double in the system source: 0 = SODECL's own sine (default;
-                                Cody-Waite reduction by pi and a series, at most 0.87 ulp measured,
-                                about 40% of the device library's instructions; the library's sine
+                                Cody-Waite reduction by pi and a polynomial, at most 0.88 ulp measured,
+                                about two thirds of the device library's instructions; the library's sine
                                 for |x| > 2^30 and within ~|x| 1.6e-16 of a nonzero multiple of pi),
                                 1 = the device library's sin */
 } sodecl_options;
