@@ -105,8 +105,13 @@ SODECL_MATH_FN double sodecl_sin_reduced(double x, bool* library)
     // sin x = sign(x) (-1)^k sin r
     unsigned lo;   // low word of kk (little-endian on the host and the GPU): k mod 2^32
     __builtin_memcpy(&lo, &kk, sizeof lo);
-    const unsigned flip = ((lo << 31) ^ sodecl_hi_word(x)) & 0x80000000u;
-    return sodecl_with_hi_word(s, sodecl_hi_word(s) ^ flip);
+    const unsigned sx = sodecl_hi_word(x) & 0x80000000u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned hi = __builtin_amdgcn_bitop3_b32(sodecl_hi_word(s), lo << 31, sx, 0x96);   // a ^ b ^ c
+#else
+    const unsigned hi = sodecl_hi_word(s) ^ (lo << 31) ^ sx;
+#endif
+    return sodecl_with_hi_word(s, hi);
 }
 
 SODECL_MATH_FN double sodecl_sin(double x)
