@@ -298,6 +298,16 @@ const char* sodecl_last_error(const sodecl_ctx* ctx);
  * byte counters when profiling the integrators. */
 int sodecl_copy_device(void* dst, const void* src, size_t bytes, void* stream);
 
+/* Diagnostic (no reference counterpart): n Threefry-2x64-20 blocks computed
+ * on the device by the same functions the SDE integrators and the sampler
+ * use, so the device stream can be checked against published known-answer
+ * vectors and against the reference's SDE key (Random123/threefry.h:178-293,
+ * u01.h:129-131, src/kernels/stochastic_solver_caller.cl:31-52).  Device
+ * memory: in[n][4] = {ctr0, ctr1, key0, key1}; out[n][4] = {word0, word1,
+ * bits of u01(word0), bits of u01(word1)}, u01 the open-interval (0, 1)
+ * double of the SDE noise.  Enqueued on stream (NULL: the default stream). */
+int sodecl_threefry_device(const uint64_t* in, uint64_t* out, size_t n, void* stream);
+
 /* Replace sodeclmgr::read_binary_data_from_file / read_data_from_file
  * (sodeclmgr.hpp:344,354; sodeclmgr.cpp:839-914).  data must hold capacity
  * doubles; *count receives the number read.  Return 1/0. */

@@ -195,12 +195,18 @@ int init_ctx(sodecl_ctx* c, int solver, double dt, double tspan, int ksteps, int
     return 1;
 }
 
-void free_device(sodecl_ctx* c)
+void drop_graph(sodecl_ctx* c)
 {
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
+    c->graph_key.clear();
+}
+
+void free_device(sodecl_ctx* c)
+{
+    drop_graph(c);
     for (auto& kv : c->programs)
         if (kv.second->module) hipModuleUnload(kv.second->module);
     c->programs.clear();
@@ -531,13 +537,12 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
         // Chunk k runs on the compute stream into buffer k%2 while the copy
         // stream moves chunk k-1 to pinned host memory and the host hands
         // chunk k-2 to the sink.
-        hipEvent_t kdone[2], cdone[2];
-        for (int q = 0; q < 2; ++q) {
-            hipEventCreateWithFlags(&kdone[q], hipEventDisableTiming);
-            hipEventCreateWithFlags(&cdone[q], hipEventDisableTiming);
-        }
-        size_t rows_of[2] = {0, 0};
+        hipEvent_t kdone[2] = {nullptr, nullptr}, cdone[2] = {nullptr, nullptr};
         bool ok = true;
+        for (int q = 0; ok && q < 2; ++q)
+            ok = hip_ok(c, hipEventCreateWithFlags(&kdone[q], hipEventDisableTiming), "hipEventCreate") &&
+                 hip_ok(c, hipEventCreateWithFlags(&cdone[q], hipEventDisableTiming), "hipEventCreate");
+        size_t rows_of[2] = {0, 0};
         long k = 0;
         for (long s = 0; ok && s < launches; s += per, ++k) {
             const int b = (int)(k & 1);
@@ -570,8 +575,8 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
         }
         if (ok) ok = hip_ok(c, hipStreamSynchronize(c->stream), "kernel");
         for (int q = 0; q < 2; ++q) {
-            hipEventDestroy(kdone[q]);
-            hipEventDestroy(cdone[q]);
+            if (kdone[q]) hipEventDestroy(kdone[q]);
+            if (cdone[q]) hipEventDestroy(cdone[q]);
         }
         if (!ok) return 0;
     }
@@ -616,7 +621,10 @@ std::string tune_cache_path(const sodecl_ctx* c)
 {
     // keyed by everything that selects the programs (text of one candidate)
     ProgramKey k = sodecl_amd::make_key(c, 0, 0, 0, std::string());
-    std::string text = assemble_program(c->system_source, k, c->features) + device_arch(c->device);
+    // the pairwise option selects the group candidates' LDS layouts, which
+    // the schedule-0 program text does not show
+    std::string text = assemble_program(c->system_source, k, c->features) + device_arch(c->device) +
+                       ":pairwise=" + std::to_string(c->opts.pairwise);
     uint64_t h = 1469598103934665603ull;
     for (unsigned char ch : text) {
         h ^= ch;
@@ -648,7 +656,10 @@ int autotune(sodecl_ctx* c)
     {
         std::ifstream f(path);
         int s = 0, g = 0, b = 0;
-        if (f >> s >> g >> b && (s == 1 || s == 2 || s == 4) && b > 0) {
+        if (f >> s >> g >> b && (s == 1 || s == 2 || s == 4) && b > 0 && b <= 1024 && g >= 1 && g <= 64) {
+            // a cached group block never exceeds what the pairwise layout's
+            // LDS allows now
+            if (s == 2) b = std::max(g, std::min(b, pairs_block_cap(c, g)) / g * g);
             c->tuned_schedule = s;
             c->tuned_group = g;
             c->tuned_block = b;
@@ -679,9 +690,16 @@ int autotune(sodecl_ctx* c)
         }
         float ms_best = 1e30f;
         for (int rep_i = 0; rep_i < 2; ++rep_i) {   // the first launch also loads the code object
-            hipMemcpyAsync(t, c->d_t, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
-            hipMemcpyAsync(y, c->d_y, n * c->neq * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
-            if (cnt) hipMemcpyAsync(cnt, c->d_counters, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+            if (!hip_ok(c, hipMemcpyAsync(t, c->d_t, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                        "hipMemcpy(tune)") ||
+                !hip_ok(c, hipMemcpyAsync(y, c->d_y, n * c->neq * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                        "hipMemcpy(tune)") ||
+                (cnt && !hip_ok(c, hipMemcpyAsync(cnt, c->d_counters, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                                  c->stream),
+                                "hipMemcpy(tune)"))) {
+                release();
+                return 0;
+            }
             hipEventRecord(c->ev_start, c->stream);
             if (!enqueue(c, cfg, t, y, c->d_params, cnt, nullptr, nullptr, 0, 1, 0, c->stream)) break;
             hipEventRecord(c->ev_stop, c->stream);
@@ -754,11 +772,7 @@ int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params,
         << (void*)ckpt_y << (void*)stage_ws << ":" << ckpt_every << ":" << launches << ":" << per << ":" << cfg.schedule << ":"
         << cfg.group << ":" << cfg.block;
     if (c->graph_key != key.str() || !c->graph_exec) {
-        if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-        if (c->graph) hipGraphDestroy(c->graph);
-        c->graph_exec = nullptr;
-        c->graph = nullptr;
-        c->graph_key.clear();
+        drop_graph(c);
         // compile before recording: no JIT inside a capture
         if (!get_program(c, cfg.schedule, cfg.group, cfg.block)) return 0;
         if (!hip_ok(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
@@ -831,8 +845,10 @@ void sodecl_destroy(sodecl_ctx* c)
 {
     if (!c) return;
     if (c->ready || c->stream) {
+        // run_device may have queued work on caller streams that still reads
+        // this context's modules: wait for the whole device
         hipSetDevice(c->device);
-        if (c->stream) hipStreamSynchronize(c->stream);
+        hipDeviceSynchronize();
     }
     free_device(c);
     delete c;
@@ -911,7 +927,13 @@ int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
                      o->pairwise != c->opts.pairwise || o->newton_tol != c->opts.newton_tol ||
                      o->library_sin != c->opts.library_sin;
     c->opts = *o;
-    if (recompile && c->ready) {
+    if (recompile && (c->ready || !c->programs.empty())) {
+        // Work enqueued by run_device on caller streams may still use the old
+        // modules, and a recorded graph holds their kernels: wait for the
+        // device, drop the graph, then unload.
+        if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
+        if (!hip_ok(c, hipDeviceSynchronize(), "hipDeviceSynchronize")) return 0;
+        drop_graph(c);
         for (auto& kv : c->programs)
             if (kv.second->module) hipModuleUnload(kv.second->module);
         c->programs.clear();
@@ -934,15 +956,15 @@ int sodecl_setup(sodecl_ctx* c)
     if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
     if (!hip_ok(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) return 0;
     if (!hip_ok(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate")) return 0;
-    hipEventCreate(&c->ev_start);
-    hipEventCreate(&c->ev_stop);
+    if (!hip_ok(c, hipEventCreate(&c->ev_start), "hipEventCreate")) return 0;
+    if (!hip_ok(c, hipEventCreate(&c->ev_stop), "hipEventCreate")) return 0;
     const size_t n = (size_t)c->orbits;
     if (!hip_ok(c, hipMalloc(&c->d_t, n * sizeof(double)), "hipMalloc(t0)")) return 0;
     if (!hip_ok(c, hipMalloc(&c->d_y, n * c->neq * sizeof(double)), "hipMalloc(y0)")) return 0;
     if (!hip_ok(c, hipMalloc(&c->d_params, std::max<size_t>(1, n * c->npar) * sizeof(double)), "hipMalloc(params)"))
         return 0;
     if (!hip_ok(c, hipMalloc(&c->d_newton, n * sizeof(long long)), "hipMalloc(newton)")) return 0;
-    hipMemsetAsync(c->d_newton, 0, n * sizeof(long long), c->stream);
+    if (!hip_ok(c, hipMemsetAsync(c->d_newton, 0, n * sizeof(long long), c->stream), "hipMemset(newton)")) return 0;
     std::vector<double> zeros;
     const double* t0 = c->h_t0;
     if (!t0) {
@@ -967,7 +989,7 @@ int sodecl_setup(sodecl_ctx* c)
     // compile the program the first run will use (JIT outside the timed loop)
     LaunchCfg cfg = choose_config(c);
     if (!get_program(c, cfg.schedule, cfg.group, cfg.block)) return 0;
-    hipStreamSynchronize(c->stream);
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return 0;
     c->ready = true;
     return 1;
 }

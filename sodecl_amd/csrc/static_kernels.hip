@@ -34,6 +34,35 @@ extern "C" int sodecl_copy_device(void* dst, const void* src, size_t bytes, void
     return hipGetLastError() == hipSuccess ? 1 : 0;
 }
 
+// Diagnostic: Threefry-2x64-20 blocks and their open-interval doubles,
+// computed by the same device functions the integrators and the sampler
+// compile.  in[i] = {ctr0, ctr1, key0, key1}; out[i] = {word0, word1, bits of
+// u01(word0), bits of u01(word1)}.
+__global__ void __launch_bounds__(256) sodecl_threefry_blocks(const sodecl_u64* __restrict__ in,
+                                                               sodecl_u64* __restrict__ out, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        sodecl_u64 w0, w1;
+        sodecl_threefry2x64_20(in[4 * i], in[4 * i + 1], in[4 * i + 2], in[4 * i + 3], &w0, &w1);
+        out[4 * i] = w0;
+        out[4 * i + 1] = w1;
+        out[4 * i + 2] = (sodecl_u64)__double_as_longlong(sodecl_u01(w0));
+        out[4 * i + 3] = (sodecl_u64)__double_as_longlong(sodecl_u01(w1));
+    }
+}
+
+extern "C" int sodecl_threefry_device(const uint64_t* in, uint64_t* out, size_t n, void* stream)
+{
+    if (!in || !out) return 0;
+    if (n == 0) return 1;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(sodecl_threefry_blocks, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const sodecl_u64*)in, (sodecl_u64*)out, n);
+    return hipGetLastError() == hipSuccess ? 1 : 0;
+}
+
 // Draws of sodecl_sample_device: entry (row, col) of out[rows][cols] comes
 // from one Threefry-2x64-20 block with counter {row, col} and key
 // {seed, SODECL_SAMPLE_KEY1}, so every entry is reproducible on its own

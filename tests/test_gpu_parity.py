@@ -241,22 +241,33 @@ def test_build_error_returns_empty(tmp_path):
         s.setup(np.zeros(2), np.ones(2))
 
 
-def test_reference_example_plugin_verbatim():
-    """A system written exactly in the reference's OpenCL-C style (with
-    __global-free arrays, int/double mixing, M_PI) runs unchanged."""
+def test_opencl_style_plugin_unchanged():
+    """A system source written in the reference's OpenCL-C plugin style --
+    a helper function, #define sizes, int loop counters and int/double
+    mixing, M_PI, a while loop, arrays sized by the macro contract -- runs
+    unchanged.  It is the Kuramoto model written another way (sums from the
+    last oscillator down, the phase difference shifted by a whole turn), so
+    the oracle's Kuramoto plugin checks it to the trajectory tolerance."""
     src = """
-#define NOSC _numeq_
+#define NUM_OSC _numeq_
+#define TURN (2 * M_PI)
+
+double pull_of(double from, double to)
+{
+    /* sine of the phase lead, a whole turn added: sin is 2 pi periodic */
+    return sin(from - to + TURN);
+}
+
 void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_numpar_])
 {
-    for (int i = 0; i<NOSC; i++)
-    {
-        yout[i] = 0;
-        for (int j = 0; j<NOSC; j++)
-        {
-            yout[i] = yout[i] + (sin(y[j] - y[i]));
-        }
-        yout[i] = yout[i] * (double)(p[0] / (double)(NOSC));
-        yout[i] = yout[i] + p[i+1] + 0 * M_PI;
+    int n = NUM_OSC;
+    double coupling = p[0] / n;
+    for (int a = 0; a < n; a++) {
+        double acc = 0;
+        int b = n;
+        while (b-- > 0)
+            acc += pull_of(y[b], y[a]);
+        yout[a] = p[1 + a] + coupling * acc;
     }
 }
 """
@@ -266,7 +277,7 @@ void sode_system(double t, double y[_numeq_], double yout[_numeq_], double p[_nu
     out = s.run().reshape(s.nsnap, 5, 64)
     cfg["tspan"] = 1.0
     ref = oracle_run(cfg, "kuramoto")["out"]
-    assert_traj_close(out, ref, what="verbatim plugin")
+    assert_traj_close(out, ref, what="OpenCL-C style plugin")
 
 
 def test_torch_device_run():
