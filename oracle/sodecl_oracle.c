@@ -484,7 +484,11 @@ static void orc_tangent_rhs(double t, double* x, double* p, const double* dx, do
     double (*jac)[n] = malloc(sizeof(double) * (size_t)n * (size_t)n);
     double (*jp)[np > 0 ? np : 1] = malloc(sizeof(double) * (size_t)n * (size_t)(np > 0 ? np : 1));
     (void)t;
-    calc_jacobian(x, jac, p);
+    /* the derivative of the right-hand side: the exact Jacobian where the
+     * plugin has one (the oculomotor's calc_jacobian keeps the reference's
+     * Newton-matrix sign in J[4][5], which is not df/dy) */
+    if (calc_jacobian_exact) calc_jacobian_exact(x, jac, p);
+    else calc_jacobian(x, jac, p);
     if (np > 0) calc_jacobian_params(x, jp, p);
     for (int i = 0; i < n; i++) {
         for (int k = 0; k < ncol; k++) {

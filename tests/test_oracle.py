@@ -297,6 +297,55 @@ def test_oculomotor_newton_matrix_keeps_the_reference_sign():
     assert je[4, 5] == -jn[4, 5]
 
 
+@pytest.mark.parametrize("system,solver", [("oculomotor", "euler"), ("oculomotor", "rk4"), ("kuramoto", "euler"),
+                                           ("kuramoto", "rk4")])
+def test_explicit_sensitivity_matches_central_differences(system, solver):
+    """The oracle's forward sensitivity of Euler and RK4 (the exact tangent
+    of each step) against central differences of the oracle's own
+    integration, on every initial-state and parameter component of three
+    orbits, against a random cotangent on every snapshot.  A dropped stage
+    term or a transposed Jacobian in the tangent fails it, and so did the
+    tangent's earlier use of the oculomotor's Newton matrix (which keeps the
+    reference's sign in J[4][5]) in place of df/dy: 8e-4 off for RK4."""
+    o = Oracle(system)
+    rng = np.random.default_rng(11)
+    if system == "oculomotor":
+        orbits, n = 3, 6
+        y0 = np.tile(OCU_Y0, (orbits, 1)) + rng.normal(0, 0.05, (orbits, n))
+        p = np.tile(OCU_P, (orbits, 1)) * np.exp(rng.uniform(-0.3, 0.3, (orbits, 6)))
+        # Euler needs the smaller step for these orbits to be well
+        # conditioned: at 1e-5 one of them amplifies rounding so much that
+        # central differences at h = 1e-6 and 1e-7 disagree in sign
+        dt, tspan, ks = (2e-6, 0.01, 250) if solver == "euler" else (1e-5, 0.01, 50)
+    else:
+        orbits, n = 3, 5
+        y0 = rng.uniform(0, 2 * np.pi, (orbits, n))
+        p = np.concatenate([rng.uniform(0, 3, (orbits, 1)), rng.normal(1, 0.1, (orbits, n))], 1)
+        dt, tspan, ks = 0.01, 1.0, 10
+    nsnap = o.run(solver, y0, p, dt, tspan, ks)["out"].shape[0]
+    cot = rng.normal(size=(nsnap, n, orbits))
+    gy, gp = o.sensitivity_gradient(solver, y0, p, cot, dt, tspan, ks)
+
+    def loss(yv, pv):
+        out = o.run(solver, yv, pv, dt, tspan, ks)["out"]
+        return np.einsum("sno,sno->o", out, cot)
+
+    for j in range(n):
+        h = 1e-6
+        yp, ym = y0.copy(), y0.copy()
+        yp[:, j] += h
+        ym[:, j] -= h
+        fd = (loss(yp, p) - loss(ym, p)) / (2 * h)
+        np.testing.assert_allclose(gy[:, j], fd, rtol=1e-6, atol=1e-6 * np.abs(gy).max())
+    for k in range(p.shape[1]):
+        h = 1e-6 * np.abs(p[:, k]).max()
+        pp, pm = p.copy(), p.copy()
+        pp[:, k] += h
+        pm[:, k] -= h
+        fd = (loss(y0, pp) - loss(y0, pm)) / (2 * h)
+        np.testing.assert_allclose(gp[:, k], fd, rtol=1e-6, atol=1e-6 * np.abs(gp).max())
+
+
 @pytest.mark.parametrize("system,solver", [("oculomotor", "ie"), ("oculomotor", "im"), ("kuramoto", "ie"),
                                            ("kuramoto", "im")])
 def test_implicit_sensitivity_matches_central_differences(system, solver):
