@@ -72,6 +72,45 @@ def test_graph_replay_matches_eager_launches():
     assert_traj_close(traj_g.cpu().numpy(), ref, what="graph")
 
 
+def test_graph_rerecorded_after_recompiling_option():
+    """An option that changes the program text (here library_sin) recompiles
+    the context's programs; a graph recorded before it held the old modules'
+    kernels, so the next device run with the same buffers records anew and
+    runs the new program: bit-identical to a fresh context with that option,
+    and within the parity bar of the oracle."""
+    cfg = configs.kuramoto_euler(orbits=130, n=10, seed=9)
+    cfg["solver"] = 2
+    cfg["tspan"] = 0.5
+    opts = dict(schedule="group", group_size=16, segments_per_launch=1, use_graph=1)
+    y_init = torch.tensor(cfg["y0"], device=DEV)
+    p = torch.tensor(cfg["params"], device=DEV)
+    t = torch.zeros(130, dtype=torch.float64, device=DEV)
+    y = torch.empty_like(y_init)
+
+    def device_run(session, traj):
+        y.copy_(y_init)
+        t.zero_()
+        torch.cuda.synchronize()
+        session.run_device(t.data_ptr(), y.data_ptr(), p.data_ptr(), 0, traj.data_ptr(), 0)
+        torch.cuda.synchronize()
+        return traj.clone()
+
+    s = configs.session_for(cfg, **opts)
+    s.setup(cfg["y0"], cfg["params"])
+    traj = torch.empty((s.nsnap, 10, 130), dtype=torch.float64, device=DEV)
+    first = device_run(s, traj)
+    s.set_options(dict(opts, library_sin=1))
+    second = device_run(s, traj)
+    fresh = configs.session_for(cfg, **dict(opts, library_sin=1))
+    fresh.setup(cfg["y0"], cfg["params"])
+    want = device_run(fresh, torch.empty_like(traj))
+    assert torch.equal(second, want)
+    assert not torch.equal(first, second)
+    ref = Oracle("kuramoto").run("rk4", cfg["y0"], cfg["params"], cfg["dt"], cfg["tspan"], cfg["ksteps"])["out"]
+    assert_traj_close(first.cpu().numpy(), ref, what="before the option change")
+    assert_traj_close(second.cpu().numpy(), ref, what="after the option change")
+
+
 @pytest.mark.parametrize("solver,n,group", [(1, 7, 8), (2, 10, 4), (2, 32, 32), (2, 64, 64), (0, 32, 16), (0, 5, 8),
                                             (1, 10, 64), (2, 7, 32), (0, 6, 32)])
 def test_pairwise_drift_bit_identical(solver, n, group):

@@ -219,6 +219,11 @@ def test_file_output(tmp_path):
         data = np.fromfile(tmp_path / "sodecloutput.bin", dtype=np.float64)
     finally:
         os.chdir(cwd)
+    # the file holds nsnap x neq x orbits raw doubles in the reference's
+    # [snap][eq][orbit] order: read that way it is the oracle's trajectory
+    ref = oracle_run(cfg, "oculomotor")["out"]
+    assert data.size == ref.size
+    assert_traj_close(data.reshape(ref.shape), ref, what="sodecloutput.bin")
     np.testing.assert_array_equal(data, arr)
 
 
