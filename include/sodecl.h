@@ -29,8 +29,9 @@
 extern "C" {
 #endif
 
-/* 2: sodecl_options gained library_sin (callers compiled against 1 pass a shorter struct) */
-#define SODECL_AMD_ABI_VERSION 2
+/* 2: sodecl_options gained library_sin (callers compiled against 1 pass a shorter struct)
+ * 3: sodecl_options gained library_exp */
+#define SODECL_AMD_ABI_VERSION 3
 
 /* include/solver_Type.hpp:14-20 */
 enum sodecl_solver {
@@ -98,6 +99,10 @@ typedef struct sodecl_options {
                                 about two thirds of the device library's instructions; the library's sine
                                 for |x| > 2^30 and within ~|x| 1.6e-16 of a nonzero multiple of pi),
                                 1 = the device library's sin */
+    int library_exp;         /* exp() of a double in the system source: 0 = SODECL's own exponential
+                                (default; Tang's table-driven method with 256 points, at most 0.68 ulp
+                                measured, fewer instructions than the device library's), 1 = the device
+                                library's exp */
 } sodecl_options;
 
 typedef struct sodecl_ctx sodecl_ctx;

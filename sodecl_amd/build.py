@@ -49,7 +49,7 @@ def embed_device_sources():
     """device/*.hip -> _embedded_device.inc (C++ raw strings): the prelude
     emitted before the system source and the integrators emitted after it."""
     out = os.path.join(CSRC, "_embedded_device.inc")
-    new = (_raw_string("kPreludeSource", ["sin.hip", "prelude.hip"]) + _raw_string("kIntegratorsSource", DEVICE_SOURCES) +
+    new = (_raw_string("kPreludeSource", ["sin.hip", "exp.hip", "prelude.hip"]) + _raw_string("kIntegratorsSource", DEVICE_SOURCES) +
            _raw_string("kBatchClassSource", ["batch_class.hip"]) +
            _raw_string("kBatchKernelSource", ["batch_kernel.hip"]))
     if not os.path.exists(out) or open(out).read() != new:

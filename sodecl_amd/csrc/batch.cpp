@@ -81,6 +81,7 @@ ProgramKey batch_key(const sodecl_batch* b)
     k.newton_tol = b->opts.newton_tol;
     k.fp_contract = b->opts.fp_contract;
     k.library_sin = b->opts.library_sin;
+    k.library_exp = b->opts.library_exp;
     return k;
 }
 

@@ -115,11 +115,19 @@ std::string epsilon_define(const ProgramKey& k)
     return buf;
 }
 
-// sin() of the system source resolves to sodecl_sin (device/sin.hip) unless
-// the caller asked for the device library's; the solver templates after the
-// system source keep the library's functions.
-std::string system_sin_begin(const ProgramKey& k) { return k.library_sin ? "" : "#define sin sodecl_sin\n"; }
-std::string system_sin_end(const ProgramKey& k) { return k.library_sin ? "" : "#undef sin\n"; }
+// sin() and exp() of the system source resolve to sodecl_sin (device/sin.hip)
+// and sodecl_exp (device/exp.hip) unless the caller asked for the device
+// library's; the solver templates after the system source keep the library's
+// functions.
+std::string system_sin_begin(const ProgramKey& k)
+{
+    return std::string(k.library_sin ? "" : "#define sin sodecl_sin\n") +
+           (k.library_exp ? "" : "#define exp sodecl_exp\n");
+}
+std::string system_sin_end(const ProgramKey& k)
+{
+    return std::string(k.library_sin ? "" : "#undef sin\n") + (k.library_exp ? "" : "#undef exp\n");
+}
 
 std::string assemble_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f)
 {

@@ -47,6 +47,7 @@ void apply_options(sodecl_options& o, const py::dict& d)
         else if (k == "pairwise") o.pairwise = kv.second.cast<int>();
         else if (k == "newton_tol") o.newton_tol = kv.second.cast<double>();
         else if (k == "library_sin") o.library_sin = kv.second.cast<int>();
+        else if (k == "library_exp") o.library_exp = kv.second.cast<int>();
         else throw std::invalid_argument("unknown option " + k);
     }
 }

@@ -113,6 +113,7 @@ ProgramKey make_key(const sodecl_ctx* ctx, int schedule, int group, int block, c
     k.newton_tol = ctx->opts.newton_tol;
     k.fp_contract = ctx->opts.fp_contract;
     k.library_sin = ctx->opts.library_sin;
+    k.library_exp = ctx->opts.library_exp;
     k.use_pairs = schedule == 2 ? pairs_layout(ctx, group) : 0;
     // At most one wavefront per SIMD: tell the compiler so.  It may then use
     // the whole register file and schedule for instruction-level
@@ -922,10 +923,11 @@ int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
     if (!(o->newton_tol >= 0)) return fail(c, "newton_tol must be non-negative");
     if (o->pairwise < -1 || o->pairwise > 4) return fail(c, "pairwise must be in -1..4");
     if (o->library_sin < 0 || o->library_sin > 1) return fail(c, "library_sin must be 0 or 1");
+    if (o->library_exp < 0 || o->library_exp > 1) return fail(c, "library_exp must be 0 or 1");
     // options that change the program text
     bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract ||
                      o->pairwise != c->opts.pairwise || o->newton_tol != c->opts.newton_tol ||
-                     o->library_sin != c->opts.library_sin;
+                     o->library_sin != c->opts.library_sin || o->library_exp != c->opts.library_exp;
     c->opts = *o;
     if (recompile && (c->ready || !c->programs.empty())) {
         // Work enqueued by run_device on caller streams may still use the old

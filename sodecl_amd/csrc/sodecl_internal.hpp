@@ -46,6 +46,7 @@ struct ProgramKey {
     int use_pairs = 0;           // pairwise drift under the group schedule: 0 off, 1 full matrix, 2 half storage
     int one_wave_per_simd = 0;   // the launch has at most one wavefront per SIMD (occupancy hint)
     int library_sin = 0;         // sin() in the system source: 0 sodecl_sin, 1 the device library's
+    int library_exp = 0;         // exp() in the system source: 0 sodecl_exp, 1 the device library's
     std::string extra_defines;   // additional "#define" lines (adjoint, batch)
 };
 
