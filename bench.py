@@ -12,6 +12,10 @@ events on its launch stream (FP64 algorithmic flops, 78.6 TFLOP/s peak);
 the reference's kernels compiled for the host CPU (oracle/_ref) timed on a
 bounded sample on all host cores; and the reference's launch design on the
 same GPU (one launch per ksteps segment with a blocking state read-back).
+The same line carries the other workloads measured the same way
+("workloads": Kuramoto N=10 Euler, 1024 orbits; Euler-Maruyama Kuramoto
+N=32, 262144 orbits), the forward+adjoint Kuramoto N=64 RK4 run, the
+batched implicit Brusselator runs and the on-device parameter sampling.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
 """
@@ -49,6 +53,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched implicit measurement")
     ap.add_argument("--no-sampling", action="store_true", help="skip the on-device parameter sampling measurement")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the Kuramoto Euler (1024 orbits) and Euler-Maruyama (262144 orbits) measurements")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run the ahead-of-time streaming copy kernel once over this many MiB "
                          "(reference point for HBM byte counters)")
@@ -56,15 +62,22 @@ def parse():
 
 
 def traffic_from_profiles(name):
-    """HBM bytes per launch of the integration kernel, from the committed
-    rocprofv3 counter summary (a line 'traffic_bytes_per_launch: N')."""
-    path = os.path.join(HERE, "profiles", f"{name}_hbm_traffic.md")
+    """HBM bytes per launch of the integration kernel as the committed
+    rocprofv3 counter summary states them (a line
+    'traffic_bytes_per_launch: N', from two --pmc runs, FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy kernel of known size).  This run does not
+    measure them: returns (bytes, source) with source naming the file, or
+    (None, None)."""
+    rel = os.path.join("profiles", f"{name}_hbm_traffic.md")
+    path = os.path.join(HERE, rel)
     if os.path.exists(path):
         with open(path) as f:
             for line in f:
                 if line.strip().startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1].strip())
-    return None
+                    return (float(line.split(":", 1)[1].strip()),
+                            f"{rel}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same command, "
+                            f"not measured by this run")
+    return None, None
 
 
 def calibration_copy(mib):
@@ -81,6 +94,92 @@ def calibration_copy(mib):
     assert lib.sodecl_copy_device(dst.data_ptr(), src.data_ptr(), n, None) == 1
     torch.cuda.synchronize()
     assert torch.equal(src, dst)
+
+
+def measure(cfg, steps, warmup, opts, label):
+    """Times `steps` complete integrations of the workload with inputs
+    resident in HBM (state, t and counters reset on the device before each),
+    wall clock around all of them and HIP events on the launch stream around
+    each integration kernel.  Returns the measurement and the session."""
+    import numpy as np
+    import torch
+    from sodecl_amd import configs, sodecl_interface as si
+    orbits, neq = cfg["orbits"], cfg["neq"]
+    launches = si.num_launches(cfg["tspan"], cfg["dt"], cfg["ksteps"])
+    nsnap = launches + 1
+    steps_per_run = launches * cfg["ksteps"]
+    dev = torch.device("cuda:0")
+    log(f"{label}: compiling and uploading")
+    sess = configs.session_for(cfg, **opts)
+    sess.setup(cfg["y0"], cfg["params"], None, cfg["counters"])
+    y_init = torch.tensor(cfg["y0"], dtype=torch.float64, device=dev).contiguous()
+    p = torch.tensor(cfg["params"], dtype=torch.float64, device=dev).contiguous()
+    y = torch.empty_like(y_init)
+    t = torch.zeros(orbits, dtype=torch.float64, device=dev)
+    cnt_init = None if cfg["counters"] is None else torch.tensor(cfg["counters"], dtype=torch.float64, device=dev)
+    cnt = None if cnt_init is None else torch.empty_like(cnt_init)
+    traj = torch.empty((nsnap, neq, orbits), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    ev = []
+
+    def one_step(record):
+        with torch.cuda.stream(stream):
+            y.copy_(y_init)
+            t.zero_()
+            if cnt is not None:
+                cnt.copy_(cnt_init)
+            if record:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            sess.run_device(t.data_ptr(), y.data_ptr(), p.data_ptr(), 0 if cnt is None else cnt.data_ptr(),
+                            traj.data_ptr(), stream.cuda_stream)
+            if record:
+                e1.record(stream)
+                ev.append((e0, e1))
+
+    for i in range(warmup):
+        one_step(False)
+        torch.cuda.synchronize()
+        log(f"{label}: warmup {i + 1}/{warmup} done")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        one_step(True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    log(f"{label}: timed {steps} steps in {wall:.3f} s")
+    kms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+    flops = cfg["flops_per_orbit_step"] * orbits * steps_per_run
+    achieved = flops / (kms * 1e-3) / 1e12
+    traffic, traffic_source = traffic_from_profiles(cfg["name"])
+    sched = sess.launch_config()
+    finite = float(np.isfinite(traj[-1].cpu().numpy()).mean())
+    res = dict(
+        value=orbits * steps_per_run / (wall / steps), ms_per_step=wall * 1e3 / steps, steps_per_run=steps_per_run,
+        nsnap=nsnap, finite=finite,
+        config={
+            "workload": cfg["name"],
+            "system": os.path.basename(cfg["system"]),
+            "solver": {0: "euler-maruyama", 1: "euler", 2: "rk4", 3: "implicit-euler", 4: "implicit-midpoint"}[
+                cfg["solver"]],
+            "orbits": orbits, "neq": neq, "dt": cfg["dt"], "steps": steps_per_run, "ksteps": cfg["ksteps"],
+            "snapshots": nsnap,
+            "schedule": {1: "lane", 2: "group", 4: "team"}.get(sched[0], str(sched[0])), "group_size": sched[1],
+            "block_size": sched[2],
+        },
+        roofline={
+            "bound": "fp64-valu",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+            "traffic_source": traffic_source,
+            "kernel_ms": kms,
+            "flops_per_orbit_step": cfg["flops_per_orbit_step"],
+        })
+    return res, sess, traj
 
 
 def cpu_baseline(cfg, orbits_sample):
@@ -275,66 +374,17 @@ def main():
 
     log(f"config {args.config}: building inputs")
     cfg = getattr(configs, args.config)()
-    orbits, neq, npar = cfg["orbits"], cfg["neq"], cfg["npar"]
-    launches = si.num_launches(cfg["tspan"], cfg["dt"], cfg["ksteps"])
-    nsnap = launches + 1
-    steps_per_run = launches * cfg["ksteps"]
+    orbits = cfg["orbits"]
     opts = {"schedule": args.schedule}
     if args.group_size:
         opts["group_size"] = args.group_size
     if args.block_size:
         opts["block_size"] = args.block_size
-
-    dev = torch.device("cuda:0")
-    log("compiling and uploading")
-    sess = configs.session_for(cfg, **opts)
-    sess.setup(cfg["y0"], cfg["params"], None, cfg["counters"])
-    y_init = torch.tensor(cfg["y0"], dtype=torch.float64, device=dev).contiguous()
-    p = torch.tensor(cfg["params"], dtype=torch.float64, device=dev).contiguous()
-    y = torch.empty_like(y_init)
-    t = torch.zeros(orbits, dtype=torch.float64, device=dev)
-    cnt_init = None if cfg["counters"] is None else torch.tensor(cfg["counters"], dtype=torch.float64, device=dev)
-    cnt = None if cnt_init is None else torch.empty_like(cnt_init)
-    traj = torch.empty((nsnap, neq, orbits), dtype=torch.float64, device=dev)
-    stream = torch.cuda.Stream()
-    ev = []
-
-    def one_step(record):
-        with torch.cuda.stream(stream):
-            y.copy_(y_init)
-            t.zero_()
-            if cnt is not None:
-                cnt.copy_(cnt_init)
-            if record:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-            sess.run_device(t.data_ptr(), y.data_ptr(), p.data_ptr(), 0 if cnt is None else cnt.data_ptr(),
-                            traj.data_ptr(), stream.cuda_stream)
-            if record:
-                e1.record(stream)
-                ev.append((e0, e1))
-
-    for i in range(args.warmup):
-        one_step(False)
-        torch.cuda.synchronize()
-        log(f"warmup {i + 1}/{args.warmup} done")
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        one_step(True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    log(f"timed {args.steps} steps in {wall:.3f} s")
-    kernel_ms = [a.elapsed_time(b) for a, b in ev]
-    ms_per_step = wall * 1e3 / args.steps
-    value = orbits * steps_per_run / (wall / args.steps)
-    kms = sum(kernel_ms) / len(kernel_ms)
-    flops = cfg["flops_per_orbit_step"] * orbits * steps_per_run
-    achieved = flops / (kms * 1e-3) / 1e12
+    head, sess, traj = measure(cfg, args.steps, args.warmup, opts, args.config)
+    value = head["value"]
+    steps_per_run = head["steps_per_run"]
     sched = sess.launch_config()
-    out = traj[-1].cpu().numpy()
-    finite = float(np.isfinite(out).mean())
+    finite = head["finite"]
 
     plain = None
     if not args.no_plain:
@@ -375,6 +425,23 @@ def main():
         log("on-device parameter sampling (262144 x 65)")
         samp = sampling_bench()
 
+    # the other explicit workloads, each with its own roofline and CPU sample
+    legs = None
+    if not args.no_legs and args.config == "oculomotor_rk4":
+        legs = []
+        for name, steps, warm, cpu_orbits in (("kuramoto_euler", 20, 3, 1024), ("kuramoto_em", 2, 1, 2048)):
+            lcfg = getattr(configs, name)()
+            leg, lsess, ltraj = measure(lcfg, steps, warm, {}, name)
+            del lsess, ltraj
+            leg["steps"] = steps
+            leg["warmup"] = warm
+            if not args.no_cpu_baseline:
+                log(f"{name}: CPU baseline (reference kernels on host cores)")
+                leg["cpu_baseline"] = cpu_baseline(lcfg, cpu_orbits)
+                if leg["cpu_baseline"]:
+                    leg["speedup_vs_cpu"] = leg["value"] / leg["cpu_baseline"]["value"]
+            legs.append(leg)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (reference kernels on host cores)")
@@ -390,37 +457,20 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded numpy PCG64 draws, sodecl_amd/configs.py)",
-        "config": {
-            "workload": cfg["name"],
-            "system": os.path.basename(cfg["system"]),
-            "solver": {0: "euler-maruyama", 1: "euler", 2: "rk4", 3: "implicit-euler", 4: "implicit-midpoint"}[
-                cfg["solver"]],
-            "orbits": orbits, "neq": neq, "dt": cfg["dt"], "steps": steps_per_run, "ksteps": cfg["ksteps"],
-            "snapshots": nsnap,
-            "schedule": {1: "lane", 2: "group", 4: "team"}.get(sched[0], str(sched[0])), "group_size": sched[1],
-            "block_size": sched[2],
-        },
-        "roofline": {
-            "bound": "fp64-valu",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": traffic_from_profiles(cfg["name"]),
-            "kernel_ms": kms,
-            "flops_per_orbit_step": cfg["flops_per_orbit_step"],
-        },
+        "config": head["config"],
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "plain_kernel": plain,
         "fwd_adjoint": adj,
         "batch_implicit": batch,
         "param_sampling": samp,
+        "workloads": legs,
         "speedup_vs_plain": None if plain is None else value / plain["value"],
         "speedup_vs_cpu": None if cpu is None else value / cpu["value"],
         "finite_fraction_final_state": finite,
