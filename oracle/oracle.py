@@ -178,10 +178,13 @@ class Oracle:
 
 
 class RefKernels:
-    """The reference's kernels compiled as C; run() mirrors run_sode_solver."""
+    """The reference's kernels compiled as C; run() mirrors run_sode_solver.
+    fast: the -O3 scalar build; simd: the orbit-vectorised build
+    (oracle/ref_simd.c, compile-time sizes, deterministic solvers only)."""
 
-    def __init__(self, system, solver, fast=False):
-        path = os.path.join(REF, f"libref_{system}_{solver}{'_fast' if fast else ''}.so")
+    def __init__(self, system, solver, fast=False, simd=False):
+        suffix = "_simd" if simd else ("_fast" if fast else "")
+        path = os.path.join(REF, f"libref_{system}_{solver}{suffix}.so")
         if not os.path.exists(path):
             raise FileNotFoundError(path)
         self.path = path
@@ -208,5 +211,5 @@ class RefKernels:
         return dict(out=out, y=y, t=t, counters=cnt, launches=int(r))
 
 
-def have_ref(system="oculomotor", solver="rk4"):
-    return os.path.exists(os.path.join(REF, f"libref_{system}_{solver}.so"))
+def have_ref(system="oculomotor", solver="rk4", simd=False):
+    return os.path.exists(os.path.join(REF, f"libref_{system}_{solver}{'_simd' if simd else ''}.so"))
