@@ -439,13 +439,14 @@ def main():
     legs = None
     if not args.no_legs and args.config == "oculomotor_rk4":
         legs = []
-        for name, steps, warm, cpu_orbits in (("kuramoto_euler", 20, 3, 1024), ("kuramoto_em", 2, 1, 2048)):
+        for name, steps, warm, cpu_orbits in (("kuramoto_euler", 20, 3, 1024), ("kuramoto_em", 2, 1, 2048),
+                                              ("oculomotor_rk4_plain", 5, 1, 0)):
             lcfg = getattr(configs, name)()
             leg, lsess, ltraj = measure(lcfg, steps, warm, {}, name)
             del lsess, ltraj
             leg["steps"] = steps
             leg["warmup"] = warm
-            if not args.no_cpu_baseline:
+            if not args.no_cpu_baseline and cpu_orbits:
                 log(f"{name}: CPU baseline (reference kernels on host cores)")
                 leg["cpu_baseline"] = cpu_baseline(lcfg, cpu_orbits)
                 if leg["cpu_baseline"]:

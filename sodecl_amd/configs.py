@@ -11,6 +11,7 @@ All draws use numpy.random.default_rng(seed) (PCG64).
                       steps, snapshot every 100, 65536 orbits; each parameter
                       log-uniform within x0.5..x2 of (120, 1.5, 0.0045, 0.05,
                       600, 9); initial state zeros plus N(0, 1e-3) noise
+  oculomotor_rk4_plain  the same with a plain-style plugin (a user's source)
   kuramoto_em         Kuramoto N=32, Euler-Maruyama, additive noise
                       sigma~U(0,0.5) per orbit, dt=1e-3, 10000 steps, snapshot
                       every 100, 262144 orbits; K~U(0,5), w_i~N(1,0.1);
@@ -50,6 +51,15 @@ def oculomotor_rk4(orbits=65536, seed=0, dt=1e-3, tspan=20.0, ksteps=100):
     return dict(name="oculomotor_rk4", system=os.path.join(SYSTEMS, "oculomotor.cl"), solver=2, neq=6,
                 npar=6, nnoi=0, orbits=orbits, dt=dt, tspan=tspan, ksteps=ksteps, y0=y0, params=params,
                 counters=None, flops_per_orbit_step=384)
+
+
+def oculomotor_rk4_plain(orbits=65536, seed=0, dt=1e-3, tspan=20.0, ksteps=100):
+    """oculomotor_rk4 with the plain-style plugin (systems/oculomotor_plain.cl:
+    branches per burst-curve side, IEEE divisions), as a user would write it."""
+    cfg = oculomotor_rk4(orbits=orbits, seed=seed, dt=dt, tspan=tspan, ksteps=ksteps)
+    cfg["name"] = "oculomotor_rk4_plain_plugin"
+    cfg["system"] = os.path.join(SYSTEMS, "oculomotor_plain.cl")
+    return cfg
 
 
 def kuramoto_em(orbits=262144, n=32, seed=0, tspan=10.0, ksteps=100):
@@ -102,6 +112,7 @@ def warm_jit_cache(arch="gfx950"):
         # system, solver, orbits, neq, npar, nnoi, dt, tspan, ksteps, options
         ("kuramoto.cl", 1, 1024, 10, 11, 0, 0.01, 10.0, 10, {}),
         ("oculomotor.cl", 2, 65536, 6, 6, 0, 1e-3, 20.0, 100, {}),
+        ("oculomotor_plain.cl", 2, 65536, 6, 6, 0, 1e-3, 20.0, 100, {}),
         ("kuramoto.cl", 0, 262144, 32, 65, 32, 1e-3, 10.0, 100, {}),
         ("kuramoto.cl", 2, 16384, 64, 65, 0, 0.01, 20.0, 10, {}),
     ]

@@ -102,6 +102,19 @@ def test_oculomotor_vs_oracle(solver, schedule):
 
 
 @pytest.mark.parametrize("solver", [1, 2])
+def test_plain_style_plugin_vs_oracle(solver):
+    """The oculomotor model written as a user would (systems/oculomotor_plain.cl:
+    a branch per burst-curve side, IEEE divisions) through the product path
+    against the oracle; it computes the reference plugin's operations, so
+    the tuned plugin's parity bar applies unchanged."""
+    cfg = small_oculomotor(solver)
+    cfg["system"] = sodecl_amd.system_path("oculomotor_plain")
+    _, out = run_session(cfg, schedule="lane")
+    ref = oracle_run(cfg, "oculomotor")
+    assert_traj_close(out, ref["out"], what=f"plain plugin solver {solver}")
+
+
+@pytest.mark.parametrize("solver", [1, 2])
 def test_plain_and_fast_bit_identical(solver):
     """The reference's launch design and the multi-segment launch run the same
     arithmetic: identical bits."""

@@ -448,3 +448,22 @@ def test_vectorised_reference_build_agrees_with_scalar():
     np.testing.assert_array_equal(a["t"], b["t"])
     scale = np.abs(a["out"]).max(axis=(0, 2), keepdims=True)
     assert np.max(np.abs(a["out"] - b["out"]) / scale) < 1e-12
+
+
+@pytest.mark.skipif(not have_ref("oculomotorplain", "rk4") or not have_ref("refoculomotor", "rk4"),
+                    reason="oracle/_ref not built")
+@pytest.mark.parametrize("solver", ["euler", "rk4"])
+def test_plain_plugin_matches_reference_example(solver):
+    """The plain-style oculomotor plugin (systems/oculomotor_plain.cl, the
+    user's way of writing the model) against the reference's
+    examples/oculomotor.cl, both through the reference kernels: identical
+    bits on the benchmark's parameter spread, burst inputs of both signs and
+    blow-ups included."""
+    from sodecl_amd import configs
+    cfg = configs.oculomotor_rk4(orbits=96, seed=12, dt=1e-4, tspan=0.05, ksteps=25)
+    y0 = cfg["y0"].copy()
+    y0[:, 5] = np.linspace(-2.0, 2.0, 96)
+    y0[0, 5], y0[1, 5] = 0.0, -0.0
+    a = RefKernels("oculomotorplain", solver).run(y0, cfg["params"], 1e-4, 0.05, 25)
+    b = RefKernels("refoculomotor", solver).run(y0, cfg["params"], 1e-4, 0.05, 25)
+    np.testing.assert_array_equal(a["out"].view(np.uint64), b["out"].view(np.uint64))
