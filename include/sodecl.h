@@ -207,6 +207,20 @@ int sodecl_run_device_checkpointed(sodecl_ctx* ctx, double* t0, double* y0, cons
 int sodecl_adjoint_device(sodecl_ctx* ctx, const double* ckpt_t, const double* ckpt_y, const double* params,
                           const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every,
                           void* stream);
+/* As sodecl_adjoint_device, with no work argument and no allocation: no
+ * reference counterpart.  For callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops, harnesses that forbid it).
+ * The workspace is the library's static scratch array of 8 Mi doubles
+ * (64 MiB) per GPU; configurations whose sodecl_adjoint_workspace_size
+ * exceeds that are rejected (return 0, sodecl_last_error says so).  The
+ * adjoint program must already be loaded on the context: a checkpointed
+ * forward run (sodecl_run_device_checkpointed) loads it.  Every call on a
+ * GPU shares that GPU's array (nothing in it needs to start at zero: each
+ * segment's rows are written before they are read), so calls that overlap
+ * on one GPU race. */
+int sodecl_adjoint_device_noalloc(sodecl_ctx* ctx, const double* ckpt_t, const double* ckpt_y, const double* params,
+                                  const double* cot, double* grad_y0, double* grad_params, int ckpt_every,
+                                  void* stream);
 /* Without recomputation (Euler and RK4, group schedule), where memory allows:
  * sodecl_run_device_stored integrates like sodecl_run_device and also writes
  * every step's stage vectors to work (sodecl_stored_workspace_size doubles,

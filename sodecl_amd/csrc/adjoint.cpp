@@ -5,6 +5,7 @@
 // checkpoint segment and sweeps it backwards (device/adjoint.hip), giving
 // dL/dy0 and dL/dparams of every orbit from dL/d(trajectory).
 #include <cmath>
+#include <string>
 
 #include "sodecl_internal.hpp"
 
@@ -81,7 +82,19 @@ int sodecl_run_device_checkpointed(sodecl_ctx* c, double* t0, double* y0, const 
 {
     if (!c) return 0;
     if (!ckpt_t || !ckpt_y || ckpt_every <= 0) return fail(c, "checkpoint buffers and interval required");
-    return run_device_impl(c, t0, y0, params, counters, traj, ckpt_t, ckpt_y, ckpt_every, stream);
+    if (!run_device_impl(c, t0, y0, params, counters, traj, ckpt_t, ckpt_y, ckpt_every, stream)) return 0;
+    // Checkpoints are taken for a backward sweep: load its program now, while
+    // the forward kernel runs, so that the sweep itself compiles and loads
+    // nothing.  A configuration without an adjoint keeps its old error state.
+    const std::string before = c->last_error;
+    int sched = 0, group = 0, block = 0;
+    std::shared_ptr<Program> prog;
+    if (check_adjoint(c, sched, group, block)) prog = get_program(c, sched, group, block, kAdjointDefines);
+    if (prog)
+        prog->get(implicit(c) ? "sodecl_lane_adjoint" : "sodecl_group_adjoint");
+    else
+        c->last_error = before;
+    return 1;
 }
 
 }  // extern "C"
@@ -92,14 +105,20 @@ namespace {
 // step's rows (written by sodecl_run_device_stored), no checkpoints.
 int launch_adjoint(sodecl_ctx* c, const double* ckpt_t, const double* ckpt_y, const double* params,
                    const double* cot, double* grad_y0, double* grad_params, double* work, int ckpt_every, int stored,
-                   void* stream)
+                   void* stream, bool loaded_only = false)
 {
     int sched = 0, group = 0, block = 0;
     if (!check_adjoint(c, sched, group, block)) return 0;
     if (stored && implicit(c)) return fail(c, "stored stage vectors are available for Euler and RK4 only");
     if (!hip_ok(c, hipSetDevice(c->device), "hipSetDevice")) return 0;
-    auto prog = get_program(c, sched, group, block, kAdjointDefines);
-    if (!prog) return 0;
+    auto prog = loaded_only ? find_program(c, sched, group, block, kAdjointDefines)
+                            : get_program(c, sched, group, block, kAdjointDefines);
+    if (!prog) {
+        if (loaded_only)
+            return fail(c, "the adjoint program is not loaded on this context: a checkpointed forward run "
+                           "(sodecl_run_device_checkpointed) loads it");
+        return 0;
+    }
     const char* kname = implicit(c) ? "sodecl_lane_adjoint" : "sodecl_group_adjoint";
     hipFunction_t fn = prog->get(kname);
     if (!fn) return fail(c, std::string("kernel ") + kname + " missing from the program");
@@ -132,6 +151,33 @@ int sodecl_adjoint_device(sodecl_ctx* c, const double* ckpt_t, const double* ckp
     if (ckpt_every <= 0) return fail(c, "ckpt_every must be positive");
     return launch_adjoint(c, ckpt_t, ckpt_y, params, cot, grad_y0, grad_params, work, ckpt_every, 0, stream);
 }
+
+}  // extern "C"
+
+namespace sodecl_amd {
+
+int adjoint_into_workspace(sodecl_ctx* c, const double* ckpt_t, const double* ckpt_y, const double* params,
+                           const double* cot, double* grad_y0, double* grad_params, double* work,
+                           size_t work_doubles, int ckpt_every, void* stream)
+{
+    if (!c) return 0;
+    if (!c->ready) return fail(c, "sodecl_setup has not succeeded");
+    if (!ckpt_t || !ckpt_y || !cot || !grad_y0 || (c->npar > 0 && (!params || !grad_params)) || !work)
+        return fail(c, "null device pointer");
+    if (ckpt_every <= 0) return fail(c, "ckpt_every must be positive");
+    const size_t need = sodecl_adjoint_workspace_size(c, ckpt_every);
+    if (need == 0) return fail(c, "this configuration has no adjoint");
+    if (need > work_doubles)
+        return fail(c, "adjoint workspace of " + std::to_string(need) + " doubles exceeds the " +
+                           std::to_string(work_doubles) +
+                           " doubles of the library's scratch array (fewer orbits or a shorter checkpoint "
+                           "interval per call)");
+    return launch_adjoint(c, ckpt_t, ckpt_y, params, cot, grad_y0, grad_params, work, ckpt_every, 0, stream, true);
+}
+
+}  // namespace sodecl_amd
+
+extern "C" {
 
 size_t sodecl_stored_workspace_size(const sodecl_ctx* c)
 {

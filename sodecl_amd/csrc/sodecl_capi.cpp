@@ -149,6 +149,14 @@ std::shared_ptr<Program> get_program(sodecl_ctx* ctx, int schedule, int group, i
     return prog;
 }
 
+std::shared_ptr<Program> find_program(sodecl_ctx* ctx, int schedule, int group, int block, const std::string& extra)
+{
+    std::ostringstream name;
+    name << schedule << ":" << group << ":" << block << ":" << extra;
+    auto it = ctx->programs.find(name.str());
+    return it != ctx->programs.end() ? it->second : nullptr;
+}
+
 }  // namespace sodecl_amd
 
 using namespace sodecl_amd;
@@ -985,6 +993,13 @@ int sodecl_setup(sodecl_ctx* c)
         if (!hip_ok(c, hipMemcpy(c->d_counters, c->counters.data(), n * sizeof(double), hipMemcpyHostToDevice),
                     "hipMemcpy(counters)"))
             return 0;
+    }
+    // the static scratch array's address on this device, resolved once per
+    // device here so that the entry point that uses it resolves nothing; a
+    // failure surfaces there, not in the entry points that never use it
+    {
+        std::string scratch_err;
+        scratch_address(c->device, scratch_err);
     }
     // the autotuner times the candidate schedules once per configuration
     if (c->opts.schedule == SODECL_SCHEDULE_AUTO && c->opts.autotune && !autotune(c)) return 0;

@@ -153,6 +153,22 @@ bool hip_ok(sodecl_ctx* ctx, hipError_t e, const char* what);
 // Program for the given schedule on the context's device (compiled on first use).
 std::shared_ptr<Program> get_program(sodecl_ctx* ctx, int schedule, int group, int block,
                                      const std::string& extra_defines = std::string());
+// The program if it is already compiled and loaded on the context, else null
+// (no compilation, no module load).
+std::shared_ptr<Program> find_program(sodecl_ctx* ctx, int schedule, int group, int block,
+                                      const std::string& extra_defines = std::string());
+// Backward sweep of sodecl_adjoint_device into a given workspace of
+// work_doubles doubles, with the adjoint program already loaded (a
+// checkpointed forward run loads it); no allocation.
+int adjoint_into_workspace(sodecl_ctx* c, const double* ckpt_t, const double* ckpt_y, const double* params,
+                           const double* cot, double* grad_y0, double* grad_params, double* work,
+                           size_t work_doubles, int ckpt_every, void* stream);
+// Device address of the library's static scratch array on device (the
+// device becomes current); resolved once per device.  Returns null and sets
+// err on failure.
+double* scratch_address(int device, std::string& err);
+// Doubles in the static scratch array.
+size_t scratch_doubles();
 // stage_ws: when not null, the group kernel also writes every step's stage
 // vectors there (the adjoint's whole-run workspace; group schedule only).
 int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,

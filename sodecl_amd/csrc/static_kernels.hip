@@ -3,9 +3,72 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <mutex>
+#include <string>
 
 #include "../../include/sodecl.h"
 #include "device/random.hip"
+#include "sodecl_internal.hpp"
+
+// ---------------------------------------------------------------------------
+// Static scratch of sodecl_adjoint_device_noalloc: the backward sweep's
+// workspace for callers that cannot allocate device memory.  Every GPU the
+// library runs on holds its own copy (64 MiB) until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address): the address each device gives for it is
+// resolved once per device with hipGetSymbolAddress and used instead.
+// ---------------------------------------------------------------------------
+#define SODECL_SCRATCH_DOUBLES ((size_t)8 << 20)
+#define SODECL_SCRATCH_DEVICES 64
+
+static __device__ __align__(256) double scratch[SODECL_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex scratch_mutex;
+double* scratch_ptr[SODECL_SCRATCH_DEVICES];
+}  // namespace
+
+namespace sodecl_amd {
+
+size_t scratch_doubles() { return SODECL_SCRATCH_DOUBLES; }
+
+double* scratch_address(int device, std::string& err)
+{
+    if (device < 0 || device >= SODECL_SCRATCH_DEVICES) {
+        err = "device ordinal out of range for the scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return nullptr;
+    }
+    if (!scratch_ptr[device]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        scratch_ptr[device] = (double*)p;
+    }
+    return scratch_ptr[device];
+}
+
+}  // namespace sodecl_amd
+
+extern "C" int sodecl_adjoint_device_noalloc(sodecl_ctx* ctx, const double* ckpt_t, const double* ckpt_y,
+                                             const double* params, const double* cot, double* grad_y0,
+                                             double* grad_params, int ckpt_every, void* stream)
+{
+    if (!ctx) return 0;
+    std::string err;
+    double* work = sodecl_amd::scratch_address(ctx->device, err);
+    if (!work) return sodecl_amd::fail(ctx, err);
+    return sodecl_amd::adjoint_into_workspace(ctx, ckpt_t, ckpt_y, params, cot, grad_y0, grad_params, work,
+                                              SODECL_SCRATCH_DOUBLES, ckpt_every, stream);
+}
 
 // Plain streaming copy, 16 bytes per lane per iteration, grid-stride: the
 // reference point for reading HBM traffic counters (bytes fetched and written
