@@ -142,12 +142,12 @@ def brusselator_batch(orbits=32768, seed=0, solver=3, cells=(16, 64, 256), dt=0.
 
 
 def brusselator_flops(neq, steps, updates, solver):
-    """Algorithmic FP64 flops of the batched Brusselator run: per state and
-    Newton update 32.5 (implicit Euler) / 34.5 (implicit midpoint: + the
-    midpoint), per state and step 14.5 / 16.5 for the final residual check
-    (right-hand side 9.5, residual 3, norm 2; band Jacobian with scaling 9,
-    tridiagonal solve 8, update 1)."""
+    """Algorithmic FP64 flops of the batched Brusselator run, the count the
+    workload is specified with: per state and Newton update 30 (implicit
+    Euler) / 33 (implicit midpoint, which also forms the midpoint), per state
+    and step 12.5 / 15.5 for the residual check that ends each step's
+    iteration; a division or square root counts 1, no transcendentals."""
     neq = np.asarray(neq, dtype=np.float64)
     if solver == 3:
-        return float(np.sum(neq * (32.5 * np.asarray(updates) + 14.5 * np.asarray(steps))))
-    return float(np.sum(neq * (34.5 * np.asarray(updates) + 16.5 * np.asarray(steps))))
+        return float(np.sum(neq * (30.0 * np.asarray(updates) + 12.5 * np.asarray(steps))))
+    return float(np.sum(neq * (33.0 * np.asarray(updates) + 15.5 * np.asarray(steps))))

@@ -44,12 +44,11 @@
 #define SODECL_PLDS (_numpar_ <= 8 ? 0 : _numpar_)
 #define SODECL_ORBIT_LDS (_numeq_ + 4 * SODECL_TLEN + SODECL_PLDS)
 
-// Coefficients of row i of the reference's tridiagonal matrix.
-SODECL_INLINE double sodecl_tri_a(const double* up, int i)
-{
-    return (i >= 1 && i < _numeq_ - 2) ? up[SODECL_TI(i)] : 0.0;
-}
-SODECL_INLINE double sodecl_tri_c(const double* lo, int i) { return (i < _numeq_ - 1) ? lo[SODECL_TI(i)] : 0.0; }
+// Coefficients of row i of the reference's tridiagonal matrix.  The band
+// rows are written with the reference's zeros in place (a_i = 0 outside
+// 1 <= i < N-2, c_{N-1} = 0), so reading them needs no test.
+SODECL_INLINE double sodecl_tri_a(const double* up, int i) { return up[SODECL_TI(i)]; }
+SODECL_INLINE double sodecl_tri_c(const double* lo, int i) { return lo[SODECL_TI(i)]; }
 
 // 1/b from the hardware reciprocal and two Newton-Raphson steps (within an
 // ulp); this solve is not the reference's sweep, so only rounding changes.
@@ -200,37 +199,43 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
         C = has_next ? cr * g_nx : 0.0;
         D = v[SODECL_TI(r1)] - ar * e_last - (has_next ? cr * e_nx : 0.0);
     }
-    // parallel cyclic reduction over the reduced rows.  Each lane shares its
-    // row scaled by its pivot's reciprocal (A/B, C/B, D/B), so one
-    // reciprocal per lane and level and three values per direction suffice:
-    // with the neighbours m = lane - s and p = lane + s,
-    //   A' = -A (A/B)_m,  C' = -C (C/B)_p,
-    //   B' = B - A (C/B)_m - C (A/B)_p,  D' = D - A (D/B)_m - C (D/B)_p;
-    // rows past either end contribute nothing (shared values 0).
+    // When every reduced row is already decoupled (A = C = 0: the partition
+    // boundaries fall where the matrix has no coupling, as for a system of
+    // 2x2 blocks such as the Brusselator chain under the reference's index
+    // mapping), cyclic reduction would leave B and D unchanged: skip it.
+    if (!__all(A == 0.0 && C == 0.0)) {
+        // parallel cyclic reduction over the reduced rows.  Each lane shares its
+        // row scaled by its pivot's reciprocal (A/B, C/B, D/B), so one
+        // reciprocal per lane and level and three values per direction suffice:
+        // with the neighbours m = lane - s and p = lane + s,
+        //   A' = -A (A/B)_m,  C' = -C (C/B)_p,
+        //   B' = B - A (C/B)_m - C (A/B)_p,  D' = D - A (D/B)_m - C (D/B)_p;
+        // rows past either end contribute nothing (shared values 0).
 #pragma unroll
-    for (int lv = 0; lv < SODECL_PCR_LEVELS; ++lv) {
-        const int s = 1 << lv;
-        const double iB = sodecl_rcp(B);
-        const double a = A * iB, c = C * iB, d = D * iB;
-        double am, cm, dm, ap, cp, dp;
-        if (SODECL_LPO == 64 && s == 32) {
-            sodecl_pcr_pair32(a, lane, am, ap);
-            sodecl_pcr_pair32(c, lane, cm, cp);
-            sodecl_pcr_pair32(d, lane, dm, dp);
-        } else {
-            am = sodecl_pcr_neighbour(a, lane, s, true);
-            cm = sodecl_pcr_neighbour(c, lane, s, true);
-            dm = sodecl_pcr_neighbour(d, lane, s, true);
-            ap = sodecl_pcr_neighbour(a, lane, s, false);
-            cp = sodecl_pcr_neighbour(c, lane, s, false);
-            dp = sodecl_pcr_neighbour(d, lane, s, false);
+        for (int lv = 0; lv < SODECL_PCR_LEVELS; ++lv) {
+            const int s = 1 << lv;
+            const double iB = sodecl_rcp(B);
+            const double a = A * iB, c = C * iB, d = D * iB;
+            double am, cm, dm, ap, cp, dp;
+            if (SODECL_LPO == 64 && s == 32) {
+                sodecl_pcr_pair32(a, lane, am, ap);
+                sodecl_pcr_pair32(c, lane, cm, cp);
+                sodecl_pcr_pair32(d, lane, dm, dp);
+            } else {
+                am = sodecl_pcr_neighbour(a, lane, s, true);
+                cm = sodecl_pcr_neighbour(c, lane, s, true);
+                dm = sodecl_pcr_neighbour(d, lane, s, true);
+                ap = sodecl_pcr_neighbour(a, lane, s, false);
+                cp = sodecl_pcr_neighbour(c, lane, s, false);
+                dp = sodecl_pcr_neighbour(d, lane, s, false);
+            }
+            const double nB = B - A * cm - C * ap;
+            const double nD = D - A * dm - C * dp;
+            A = -A * am;
+            C = -C * cp;
+            B = nB;
+            D = nD;
         }
-        const double nB = B - A * cm - C * ap;
-        const double nD = D - A * dm - C * dp;
-        A = -A * am;
-        C = -C * cp;
-        B = nB;
-        D = nD;
     }
     const double X = D * sodecl_rcp(B);
     const double Xm = sodecl_pcr_neighbour(X, lane, 1, true);
@@ -355,8 +360,9 @@ __device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__
                 if (SODECL_OWNS(comp)) {
                     double lo, di, up;
                     calc_jacobian_band_i(comp, sh_g, sh_p, &lo, &di, &up);
-                    sh_lo[SODECL_TI(comp)] = lo * scale * _m_dt_;
-                    sh_up[SODECL_TI(comp)] = up * scale * _m_dt_;
+                    // a_i = J[i][i+1] for 1 <= i < N-2, c_i = J[i+1][i] for i < N-1, else 0
+                    sh_lo[SODECL_TI(comp)] = comp < _numeq_ - 1 ? lo * scale * _m_dt_ : 0.0;
+                    sh_up[SODECL_TI(comp)] = (comp >= 1 && comp < _numeq_ - 2) ? up * scale * _m_dt_ : 0.0;
                     sh_di[SODECL_TI(comp)] = di * scale * _m_dt_ + 1;
                     sh_r[SODECL_TI(comp)] = r[c];
                 }
