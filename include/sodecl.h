@@ -30,8 +30,9 @@ extern "C" {
 #endif
 
 /* 2: sodecl_options gained library_sin (callers compiled against 1 pass a shorter struct)
- * 3: sodecl_options gained library_exp */
-#define SODECL_AMD_ABI_VERSION 3
+ * 3: sodecl_options gained library_exp
+ * 4: sodecl_options gained host_share */
+#define SODECL_AMD_ABI_VERSION 4
 
 /* include/solver_Type.hpp:14-20 */
 enum sodecl_solver {
@@ -103,6 +104,13 @@ typedef struct sodecl_options {
                                 (default; Tang's table-driven method with 256 points, at most 0.68 ulp
                                 measured, fewer instructions than the device library's), 1 = the device
                                 library's exp */
+    double host_share;       /* fraction of the orbits sodecl_run integrates on the host's cores while the
+                                GPU integrates the rest (0 = none, below 1): the last
+                                floor(orbits * host_share) orbits, with the same lane integrator and system
+                                source compiled for the host by its C++ compiler ("c++", or
+                                SODECL_AMD_HOST_CXX) at setup.  Applies to sodecl_run when the trajectory
+                                fits one device buffer; device runs integrate every orbit on the GPU.  The
+                                reference names such a CPU+GPU split as future work (README.md:147) */
 } sodecl_options;
 
 typedef struct sodecl_ctx sodecl_ctx;
@@ -290,6 +298,19 @@ const char* sodecl_batch_last_error(const sodecl_batch* batch);
  * time of the last run measured with HIP events on the launch stream. */
 double sodecl_last_runtime(const sodecl_ctx* ctx);
 double sodecl_last_kernel_ms(const sodecl_ctx* ctx);
+/* Host share of the last sodecl_run: wall milliseconds of the host orbits'
+ * integration and their number (0 without a host share). */
+double sodecl_last_host_ms(const sodecl_ctx* ctx, int* host_orbits);
+/* Builds (or finds in the cache) the host-share library of this context's
+ * configuration and writes its file name to path (capacity bytes); returns
+ * the name's length, 0 on failure.  The library exports
+ *   void sodecl_host_lane(double* t, double* y, const double* params,
+ *                         double* counters, double* traj, long long* newton,
+ *                         int orbits, int first, int last, int s0, int nseg,
+ *                         int final_snap, long long gstep0, int nthreads)
+ * integrating orbits [first, last) of host arrays in the layouts above for
+ * nseg ksteps segments, snapshots into traj[s0 ..][neq][orbits]. */
+size_t sodecl_host_library(sodecl_ctx* ctx, char* path, size_t capacity);
 
 /* Schedule, group size and block size the last run (or sodecl_compile_only) used. */
 int sodecl_get_launch_config(const sodecl_ctx* ctx, int* schedule, int* group_size, int* block_size);

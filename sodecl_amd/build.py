@@ -22,7 +22,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 # Every HIP compile: no packed FP32 instructions.
 NO_PK_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 DEVICE_SOURCES = ["random.hip", "integrators.hip", "adjoint.hip"]
-HOST_SOURCES = ["jit.cpp", "sodecl_capi.cpp", "sodeclmgr.cpp", "adjoint.cpp", "batch.cpp"]
+HOST_SOURCES = ["jit.cpp", "sodecl_capi.cpp", "sodeclmgr.cpp", "adjoint.cpp", "batch.cpp", "host_share.cpp"]
 
 
 def run(cmd, **kw):
@@ -50,6 +50,8 @@ def embed_device_sources():
     emitted before the system source and the integrators emitted after it."""
     out = os.path.join(CSRC, "_embedded_device.inc")
     new = (_raw_string("kPreludeSource", ["sin.hip", "exp.hip", "prelude.hip"]) + _raw_string("kIntegratorsSource", DEVICE_SOURCES) +
+           _raw_string("kHostMathSource", ["sin.hip", "exp.hip"]) +
+           _raw_string("kHostLaneSource", ["random.hip", "integrators.hip"]) +
            _raw_string("kBatchClassSource", ["batch_class.hip"]) +
            _raw_string("kBatchKernelSource", ["batch_kernel.hip"]))
     if not os.path.exists(out) or open(out).read() != new:
@@ -71,8 +73,11 @@ def build_library():
          os.path.join(CSRC, "static_kernels.hip"), "-o", hip_obj])
     objs.append(hip_obj)
     lib = os.path.join(HERE, "libsodecl_amd.so")
-    run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", lib, f"-L{ROCM}/lib", "-lhiprtc",
+    # linked under a temporary name and renamed: a reader never sees a
+    # partly written library
+    run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", lib + ".tmp", f"-L{ROCM}/lib", "-lhiprtc",
          "-lamdhip64", "-ldl", f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(lib + ".tmp", lib)
     return lib
 
 
@@ -82,7 +87,8 @@ def build_pybind():
     out = os.path.join(HERE, "sodecl_interface" + suffix)
     py_inc = sysconfig.get_paths()["include"]
     run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", f"-I{pybind11.get_include()}", f"-I{py_inc}",
-         os.path.join(CSRC, "pybind_module.cpp"), f"-L{HERE}", "-lsodecl_amd", "-Wl,-rpath,$ORIGIN", "-o", out])
+         os.path.join(CSRC, "pybind_module.cpp"), f"-L{HERE}", "-lsodecl_amd", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
     return out
 
 

@@ -51,6 +51,7 @@ sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict
                      double* __restrict__ grad_y0, double* __restrict__ grad_p, double* __restrict__ work,
                      int orbits, int total_steps, int ckpt_every, int stored)
 {
+    SODECL_EXP_TABLE_INIT();
     __shared__ double sh_x[2][SODECL_OPB][SODECL_XSTRIDE];
     __shared__ double sh_v[2][SODECL_OPB][SODECL_XSTRIDE];
     __shared__ double sh_l[2][SODECL_OPB][SODECL_XSTRIDE];
@@ -254,6 +255,7 @@ sodecl_lane_adjoint(const double* __restrict__ ckpt_t, const double* __restrict_
                     double* __restrict__ grad_y0, double* __restrict__ grad_p, double* __restrict__ work,
                     int orbits, int total_steps, int ckpt_every, int stored)
 {
+    SODECL_EXP_TABLE_INIT();
     (void)stored;   // always recomputes: stored rows exist for the explicit solvers only
     const int i = blockIdx.x * SODECL_BLOCK + threadIdx.x;
     if (i >= orbits) return;

@@ -20,6 +20,7 @@ sodecl_batch(const int* __restrict__ items, const int* __restrict__ cls, const i
              const double* __restrict__ params, double* __restrict__ out, long long* __restrict__ newton,
              int* __restrict__ head, int nitems)
 {
+    SODECL_EXP_TABLE_INIT();
     extern __shared__ double sodecl_lds[];
     for (;;) {
         int item = 0;

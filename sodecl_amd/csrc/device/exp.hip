@@ -171,6 +171,35 @@ SODECL_EXP_TABLE_QUAL double sodecl_exp_table[512] __attribute__((aligned(16))) 
     0x1.fd3c22b8f71f1p+0, 0x1.2eb74966579e7p-57, 0x1.fe9d96b2a23d9p+0, 0x1.4a6037442fde3p-56,
 };
 
+// In a program whose system source calls exp (SODECL_EXP_IN_LDS), every
+// kernel first copies the table into LDS (SODECL_EXP_TABLE_INIT, all threads
+// of the block, then a barrier) and the lookups read it there: a launch with
+// one wavefront per SIMD has no other wavefront to cover a load's latency,
+// and an LDS read returns sooner than one through the vector caches.
+#if defined(__HIP_DEVICE_COMPILE__) && defined(SODECL_EXP_IN_LDS) && SODECL_EXP_IN_LDS
+__shared__ double2 sodecl_exp_lds[256];
+__device__ inline void sodecl_exp_table_init()
+{
+    const double2* src = (const double2*)sodecl_exp_table;
+    for (int k = (int)threadIdx.x; k < 256; k += (int)blockDim.x) sodecl_exp_lds[k] = src[k];
+    __syncthreads();
+}
+#define SODECL_EXP_TABLE_INIT() sodecl_exp_table_init()
+#define SODECL_EXP_ENTRY(j, hi, lo) \
+    do {                            \
+        const double2 e2 = sodecl_exp_lds[j]; \
+        hi = e2.x;                  \
+        lo = e2.y;                  \
+    } while (0)
+#else
+#define SODECL_EXP_TABLE_INIT() ((void)0)
+#define SODECL_EXP_ENTRY(j, hi, lo) \
+    do {                            \
+        hi = sodecl_exp_table[2 * (j)];     \
+        lo = sodecl_exp_table[2 * (j) + 1]; \
+    } while (0)
+#endif
+
 SODECL_MATH_FN double sodecl_exp(double x)
 {
     const double xc = fmin(fmax(x, -1400.0), 1400.0);
@@ -184,8 +213,8 @@ SODECL_MATH_FN double sodecl_exp(double x)
     const int m = lo >> 8;
     const double q = fma(r, fma(r, 0x1.5555565c3ff25p-5, 0x1.555556dfb5410p-3), 0.5);
     const double p = fma(r * r, q, r);   // exp(r) - 1
-    const double thi = sodecl_exp_table[2 * j];
-    const double tlo = sodecl_exp_table[2 * j + 1];
+    double thi, tlo;
+    SODECL_EXP_ENTRY(j, thi, tlo);
     const double e = ldexp(thi + fma(thi, p, tlo), m);
     // NaN in, NaN out: a high word of 0x7ff80000 makes a NaN whatever the low word
     return sodecl_with_hi_word(e, __builtin_isnan(x) ? 0x7ff80000u : sodecl_hi_word(e));

@@ -9,6 +9,10 @@
 //   SODECL_BLOCK           threads per block
 //   SODECL_GROUP           lanes per orbit in the group schedule
 //   SODECL_NEWTON_MAXIT    Newton iteration cap (ie.cl:69 uses 1000)
+//   SODECL_STRICT          1: strict arithmetic (fp_contract = 1): no
+//                          contraction and correctly rounded divisions in
+//                          the solver steps; 0: contraction, and divisions
+//                          by constants as products with the reciprocal
 //   SODECL_HAS_*           optional plugin entry points found in the source
 //
 // Schedules:
@@ -59,6 +63,16 @@ SODECL_INLINE double sodecl_div(double a, double d, double rd)
     const double r = fmax(fma(q0, d, -a), -0x1.fffffffffffffp+1023);
     return fma(-r, rd, q0);
 }
+
+// (a dt) / 6.00 of the RK4 update (rk4.cl:60): under strict arithmetic the
+// correctly rounded quotient, otherwise the product with the rounded
+// reciprocal (within an ulp of it; the same size of rounding change as
+// contraction, and right for infinities, NaN and signed zeros).
+#if SODECL_STRICT
+#define SODECL_RK4_DIV6(a) sodecl_div((a), 6.00, 1.0 / 6.00)
+#else
+#define SODECL_RK4_DIV6(a) ((a) * (1.0 / 6.00))
+#endif
 
 // The reference keeps the per-orbit counter as a double and assigns it to the
 // 64-bit counter word (stochastic_solver_caller.cl:49).  Non-negative values
@@ -234,7 +248,7 @@ SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
     }
     SODECL_RHS(t + _m_dt_, x, k, p);
     SODECL_UNROLL
-    for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + sodecl_div((yout[i] + k[i]) * _m_dt_, 6.00, 1.0 / 6.00);
+    for (int i = 0; i < _numeq_; ++i) yout[i] = y[i] + SODECL_RK4_DIV6((yout[i] + k[i]) * _m_dt_);
     return 0;
 }
 #elif SODECL_IS_IMPLICIT
@@ -253,17 +267,14 @@ SODECL_INLINE int sodecl_lane_step(double t, double* y, double* yout, double* p)
 //   newton: per-orbit Newton update counts (implicit solvers) or null.
 // ---------------------------------------------------------------------------
 #if SODECL_SCHEDULE == 1
-extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
-sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
-            double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
-            int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
-            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0, double* __restrict__ stage_ws,
-            long long ws_steps)
+// Orbit i of a lane-schedule launch: one lane of sodecl_lane, or one
+// iteration of the host share's orbit loop (SODECL_HOST_BUILD).
+SODECL_INLINE void sodecl_lane_orbit(int i, double* __restrict__ t_state, double* __restrict__ y_state,
+                                     const double* __restrict__ params, double* __restrict__ counters,
+                                     double* __restrict__ traj, long long* __restrict__ newton, int orbits, int s0,
+                                     int nseg, int final_snap, double* __restrict__ ckpt_t,
+                                     double* __restrict__ ckpt_y, int ckpt_every, long long gstep0)
 {
-    const int i = blockIdx.x * SODECL_BLOCK + threadIdx.x;
-    if (i >= orbits) return;
-    (void)stage_ws;
-    (void)ws_steps;
     double y[_numeq_], yn[_numeq_], p[SODECL_NPAR_ALLOC];
     SODECL_UNROLL
     for (int m = 0; m < _numeq_; ++m) y[m] = y_state[(size_t)i * _numeq_ + m];
@@ -334,6 +345,36 @@ sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const do
 #endif
     if (newton) newton[i] += nu;
 }
+
+#if SODECL_HOST_BUILD
+// The host share: orbits [first, last) of the same arrays (host memory, the
+// full orbit count's layouts), on the calling thread and its OpenMP team.
+extern "C" void sodecl_host_lane(double* t_state, double* y_state, const double* params, double* counters,
+                                 double* traj, long long* newton, int orbits, int first, int last, int s0, int nseg,
+                                 int final_snap, long long gstep0, int nthreads)
+{
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int i = first; i < last; ++i)
+        sodecl_lane_orbit(i, t_state, y_state, params, counters, traj, newton, orbits, s0, nseg, final_snap, nullptr,
+                          nullptr, 1, gstep0);
+}
+#else
+extern "C" __global__ void __launch_bounds__(SODECL_BLOCK) SODECL_KERNEL_ATTR
+sodecl_lane(double* __restrict__ t_state, double* __restrict__ y_state, const double* __restrict__ params,
+            double* __restrict__ counters, double* __restrict__ traj, long long* __restrict__ newton,
+            int orbits, int s0, int nseg, int final_snap, double* __restrict__ ckpt_t,
+            double* __restrict__ ckpt_y, int ckpt_every, long long gstep0, double* __restrict__ stage_ws,
+            long long ws_steps)
+{
+    SODECL_EXP_TABLE_INIT();
+    const int i = blockIdx.x * SODECL_BLOCK + threadIdx.x;
+    if (i >= orbits) return;
+    (void)stage_ws;
+    (void)ws_steps;
+    sodecl_lane_orbit(i, t_state, y_state, params, counters, traj, newton, orbits, s0, nseg, final_snap, ckpt_t,
+                      ckpt_y, ckpt_every, gstep0);
+}
+#endif
 #endif  // SODECL_SCHEDULE == 1
 
 // ---------------------------------------------------------------------------
@@ -355,6 +396,7 @@ sodecl_team(double* __restrict__ t_state, double* __restrict__ y_state, const do
             double* __restrict__ ckpt_y, int ckpt_every, long long gstep0, double* __restrict__ stage_ws,
             long long ws_steps)
 {
+    SODECL_EXP_TABLE_INIT();
     const int gid = blockIdx.x * SODECL_BLOCK + threadIdx.x;
     const int i = gid / SODECL_GROUP;            // orbit: a team exits together
     const int lane = gid % SODECL_GROUP;
@@ -670,7 +712,7 @@ SODECL_INLINE void sodecl_group_step(double t, double (&y)[SODECL_CPL], double (
     SODECL_EVAL(t + _m_dt_, xs[buf][slot], f);
 #pragma unroll
     for (int c = 0; c < SODECL_CPL; ++c)
-        if (lane + c * SODECL_G < _numeq_) y[c] = y[c] + sodecl_div((acc[c] + f[c]) * _m_dt_, 6.00, 1.0 / 6.00);
+        if (lane + c * SODECL_G < _numeq_) y[c] = y[c] + SODECL_RK4_DIV6((acc[c] + f[c]) * _m_dt_);
     buf ^= 1;
 }
 #endif
@@ -687,6 +729,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
              double* __restrict__ ckpt_y, int ckpt_every, long long gstep0, double* __restrict__ stage_ws,
              long long ws_steps)
 {
+    SODECL_EXP_TABLE_INIT();
     __shared__ double sh_x[2][SODECL_OPB][SODECL_XSTRIDE];
     __shared__ double sh_p[SODECL_OPB][SODECL_NPAR_ALLOC];
 #if SODECL_IS_SDE

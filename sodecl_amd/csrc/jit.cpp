@@ -80,6 +80,7 @@ PluginFeatures detect_features(const std::string& source)
     f.team = defines("sode_system_team");
     f.jacobian_params = defines("calc_jacobian_params");
     f.jacobian_exact = defines("calc_jacobian_exact");
+    f.calls_exp = defines("exp");
     return f;
 }
 
@@ -143,6 +144,8 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     if (k.solver == SODECL_IMPLICIT_EULER || k.solver == SODECL_IMPLICIT_MIDPOINT) o << epsilon_define(k);
     // MI355X build configuration
     o << "#define SODECL_HIP 1\n";
+    o << "#define SODECL_STRICT " << (k.fp_contract ? 1 : 0) << "\n";
+    o << "#define SODECL_EXP_IN_LDS " << (!k.library_exp && f.calls_exp ? 1 : 0) << "\n";
     o << "#define SODECL_SOLVER " << k.solver << "\n";
     o << "#define SODECL_SCHEDULE " << k.schedule << "\n";
     o << "#define SODECL_GROUP " << k.group << "\n";
@@ -180,6 +183,48 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     return o.str();
 }
 
+std::string assemble_host_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f)
+{
+    std::ostringstream o;
+    o << "// SODECL host-share program (the lane integrator for the host CPU)\n";
+    o << "#include <math.h>\n#include <stddef.h>\n#include <string.h>\n";
+    o << "#define _numnoi_ " << k.nnoi << "\n";
+    o << "#define _listsize_ " << k.orbits << "\n";
+    o << "#define _numsteps_ " << k.ksteps << "\n";
+    o << "#define _m_dt_ " << dt_literal(k.dt) << "\n";
+    o << "#define _numeq_ " << k.neq << "\n";
+    o << "#define _numpar_ " << k.npar << "\n";
+    if (k.solver == SODECL_IMPLICIT_EULER || k.solver == SODECL_IMPLICIT_MIDPOINT) o << epsilon_define(k);
+    o << "#define SODECL_HOST_BUILD 1\n";
+    o << "#define SODECL_STRICT " << (k.fp_contract ? 1 : 0) << "\n";
+    o << "#define SODECL_EXP_IN_LDS 0\n";
+    o << "#define SODECL_SOLVER " << k.solver << "\n";
+    o << "#define SODECL_SCHEDULE 1\n#define SODECL_GROUP 1\n#define SODECL_BLOCK 1\n";
+    o << "#define SODECL_NEWTON_MAXIT " << k.newton_max_iter << "\n";
+    o << "#define SODECL_HAS_COMPONENT " << (f.component ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_STOCH_COMPONENT " << (f.stoch_component ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN_BAND " << (f.jacobian_band ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN_BAND_I " << (f.jacobian_band_i ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_JACOBIAN " << (f.calc_jacobian ? 1 : 0) << "\n";
+    o << "#define SODECL_HAS_VJP_GROUP 0\n#define SODECL_HAS_JACOBIAN_PARAMS 0\n#define SODECL_HAS_JACOBIAN_EXACT 0\n";
+    o << "#define SODECL_USE_PAIRS 0\n";
+    o << "#define SODECL_INLINE static inline\n#define SODECL_MATH_FN static inline\n";
+    for (const char* kw : {"__kernel", "__global", "__private"})
+        o << "#ifndef " << kw << "\n#define " << kw << "\n#endif\n";
+    o << "#ifndef __local\n#define __local\n#endif\n#ifndef __constant\n#define __constant const\n#endif\n";
+    o << "typedef unsigned long ulong;\ntypedef unsigned int uint;\ntypedef unsigned short ushort;\n"
+         "typedef unsigned char uchar;\n";
+    o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
+    o << kHostMathSource << "\n";
+    o << system_sin_begin(k);
+    o << "#line 1 \"system.cl\"\n";
+    o << system_source << "\n";
+    o << system_sin_end(k);
+    o << "#line 1 \"integrators.hip\"\n";
+    o << kHostLaneSource << "\n";
+    return o.str();
+}
+
 std::string assemble_batch_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f,
                                    const std::vector<int>& class_neq)
 {
@@ -192,6 +237,8 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
     o << "#define _numpar_ " << k.npar << "\n";
     o << epsilon_define(k);
     o << "#define SODECL_HIP 1\n";
+    o << "#define SODECL_STRICT " << (k.fp_contract ? 1 : 0) << "\n";
+    o << "#define SODECL_EXP_IN_LDS " << (!k.library_exp && f.calls_exp ? 1 : 0) << "\n";
     o << "#define SODECL_SOLVER " << k.solver << "\n";
     o << "#define SODECL_SCHEDULE 2\n";
     o << "#define SODECL_GROUP 64\n";
@@ -206,7 +253,6 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
          "typedef unsigned char uchar;\n";
     o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
     o << kPreludeSource << "\n";
-    (void)f;
     if (class_neq.size() > 1) o << "#define SODECL_BATCH_ITEM_ATTR __attribute__((noinline))\n";
     // one namespace per size class: the plugin and the per-class solver
     for (size_t c = 0; c < class_neq.size(); ++c) {

@@ -48,6 +48,7 @@ void apply_options(sodecl_options& o, const py::dict& d)
         else if (k == "newton_tol") o.newton_tol = kv.second.cast<double>();
         else if (k == "library_sin") o.library_sin = kv.second.cast<int>();
         else if (k == "library_exp") o.library_exp = kv.second.cast<int>();
+        else if (k == "host_share") o.host_share = kv.second.cast<double>();
         else throw std::invalid_argument("unknown option " + k);
     }
 }
@@ -249,6 +250,12 @@ public:
         return py::make_tuple(s, g, b);
     }
     double last_kernel_ms() const { return sodecl_last_kernel_ms(ctx_); }
+    py::tuple host_stats() const
+    {
+        int n = 0;
+        const double ms = sodecl_last_host_ms(ctx_, &n);
+        return py::make_tuple(ms, n);
+    }
     double last_runtime() const { return sodecl_last_runtime(ctx_); }
     size_t compile_only(const std::string& arch)
     {
@@ -408,6 +415,7 @@ PYBIND11_MODULE(sodecl_interface, m)
         .def("newton_updates", &Session::newton_updates)
         .def("launch_config", &Session::launch_config)
         .def("last_kernel_ms", &Session::last_kernel_ms)
+        .def("host_stats", &Session::host_stats)
         .def("last_runtime", &Session::last_runtime)
         .def("compile_only", &Session::compile_only, py::arg("arch") = "gfx950")
         .def("program_source", &Session::program_source)

@@ -26,6 +26,7 @@
 #include <iostream>
 #include <random>
 #include <sstream>
+#include <thread>
 
 #include "sodecl_internal.hpp"
 
@@ -352,14 +353,15 @@ LaunchCfg choose_config(const sodecl_ctx* c)
 int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const double* p, double* cnt, double* traj,
             long long* newton, int s0, int nseg, int final_snap, hipStream_t stream, double* ckpt_t = nullptr,
             double* ckpt_y = nullptr, int ckpt_every = 0, long long gstep0 = 0, double* stage_ws = nullptr,
-            long long ws_steps = 0)
+            long long ws_steps = 0, int norbits = -1)
 {
     auto prog = get_program(c, cfg.schedule, cfg.group, cfg.block);
     if (!prog) return 0;
     const char* kname = cfg.schedule == 2 ? "sodecl_group" : (cfg.schedule == 4 ? "sodecl_team" : "sodecl_lane");
     hipFunction_t fn = prog->get(kname);
     if (!fn) return fail(c, std::string("kernel ") + kname + " missing from the program");
-    int orbits = c->orbits;
+    // the first norbits orbits only (the rest are the host share's)
+    int orbits = norbits >= 0 ? norbits : c->orbits;
     unsigned grid;
     if (cfg.schedule == 2 || cfg.schedule == 4) {
         const int opb = cfg.block / cfg.group;
@@ -490,6 +492,95 @@ size_t traj_budget(const sodecl_ctx* c)
     return c->opts.traj_budget_bytes ? c->opts.traj_budget_bytes : ((size_t)8 << 30);
 }
 
+// Orbits of a run the host share integrates: the last floor(orbits *
+// host_share), when the host library is loaded and at least one orbit is
+// left to the GPU.
+int host_orbits(const sodecl_ctx* c)
+{
+    if (!c->host_fn || !(c->opts.host_share > 0) || c->hs_p.size() != (size_t)c->orbits * c->npar) return 0;
+    const int n = (int)((double)c->orbits * c->opts.host_share);
+    return n < c->orbits ? n : c->orbits - 1;
+}
+
+// One launch on the GPU for orbits [0, n_gpu) while the host share
+// integrates [n_gpu, orbits) into the same output rows; the host orbits'
+// final state (t, y, counters, Newton counts) goes back to the device
+// buffers so that a following run continues from it.
+int run_shared(sodecl_ctx* c, long launches, const LaunchCfg& cfg, Sink& sink, bool to_array_direct)
+{
+    const int n_host = host_orbits(c), n_gpu = c->orbits - n_host, n = c->neq;
+    const size_t slab = (size_t)n * c->orbits, nsnap = (size_t)launches + 1;
+    const bool want = c->output_type != SODECL_OUTPUT_NONE;
+    double* dst = want ? (to_array_direct ? sink.dst : c->h_stage[0]) : nullptr;
+    const size_t first = (size_t)n_gpu;
+    auto rows = [&](double* host, const double* dev, size_t width, hipMemcpyKind kind, hipStream_t st) {
+        const size_t off = first * width, cnt = (size_t)n_host * width;
+        return kind == hipMemcpyDeviceToHost
+                   ? hipMemcpyAsync(host + off, dev + off, cnt * sizeof(double), kind, st)
+                   : hipMemcpyAsync((double*)dev + off, host + off, cnt * sizeof(double), kind, st);
+    };
+    c->hs_t.resize(c->orbits);
+    c->hs_y.resize(slab);
+    c->hs_newton.assign(c->orbits, 0);
+    if (c->d_counters) c->hs_cnt.resize(c->orbits);
+    // the host orbits' current state
+    if (!hip_ok(c, rows(c->hs_t.data(), c->d_t, 1, hipMemcpyDeviceToHost, c->stream), "hipMemcpy(host share)") ||
+        !hip_ok(c, rows(c->hs_y.data(), c->d_y, (size_t)n, hipMemcpyDeviceToHost, c->stream),
+                "hipMemcpy(host share)") ||
+        (c->d_counters &&
+         !hip_ok(c, rows(c->hs_cnt.data(), c->d_counters, 1, hipMemcpyDeviceToHost, c->stream),
+                 "hipMemcpy(host share)")) ||
+        !hip_ok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+        return 0;
+    hipEventRecord(c->ev_start, c->stream);
+    if (!enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, want ? c->d_traj[0] : nullptr, c->d_newton, 0,
+                 (int)launches, 1, c->stream, nullptr, nullptr, 0, 0, nullptr, 0, n_gpu))
+        return 0;
+    hipEventRecord(c->ev_stop, c->stream);
+    // the host's orbits while the kernel runs
+    const auto h0 = std::chrono::steady_clock::now();
+    const int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    c->host_fn(c->hs_t.data(), c->hs_y.data(), c->hs_p.data(), c->d_counters ? c->hs_cnt.data() : nullptr, dst,
+               c->hs_newton.data(), c->orbits, n_gpu, c->orbits, 0, (int)launches, 1, 0, threads);
+    c->last_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    c->last_host_orbits = n_host;
+    // the GPU's columns [0, n_gpu) of every snapshot row, then the host
+    // orbits' state back to the device
+    if (want && !hip_ok(c, hipMemcpy2DAsync(dst, slab / n * sizeof(double), c->d_traj[0], (size_t)n_gpu * sizeof(double),
+                                            (size_t)n_gpu * sizeof(double), nsnap * n, hipMemcpyDeviceToHost,
+                                            c->stream),
+                         "hipMemcpy2D(trajectory)"))
+        return 0;
+    std::vector<long long> nu(c->orbits, 0);
+    if (!hip_ok(c, hipMemcpyAsync(nu.data(), c->d_newton, c->orbits * sizeof(long long), hipMemcpyDeviceToHost,
+                                  c->stream),
+                "hipMemcpy(newton)") ||
+        !hip_ok(c, hipStreamSynchronize(c->stream), "kernel"))
+        return 0;
+    for (int o = n_gpu; o < c->orbits; ++o) nu[o] += c->hs_newton[o];
+    if (!hip_ok(c, rows(c->hs_t.data(), c->d_t, 1, hipMemcpyHostToDevice, c->stream), "hipMemcpy(host share)") ||
+        !hip_ok(c, rows(c->hs_y.data(), c->d_y, (size_t)n, hipMemcpyHostToDevice, c->stream),
+                "hipMemcpy(host share)") ||
+        (c->d_counters &&
+         !hip_ok(c, rows(c->hs_cnt.data(), c->d_counters, 1, hipMemcpyHostToDevice, c->stream),
+                 "hipMemcpy(host share)")) ||
+        !hip_ok(c, hipMemcpyAsync(c->d_newton, nu.data(), c->orbits * sizeof(long long), hipMemcpyHostToDevice,
+                                  c->stream),
+                "hipMemcpy(newton)") ||
+        !hip_ok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+        return 0;
+    if (want) {
+        if (to_array_direct)
+            sink.dst += nsnap * slab;
+        else
+            sink.put(c->h_stage[0], nsnap * slab);
+    }
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    c->last_kernel_ms = ms;
+    return 1;
+}
+
 // Fast path: many segments per launch, snapshots written by the kernel.
 int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
 {
@@ -501,6 +592,7 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
     if (!want) {
         // no output: integrate in as few launches as the options allow
         long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
+        if (per >= launches && host_orbits(c) > 0) return run_shared(c, launches, cfg, sink, false);
         hipEventRecord(c->ev_start, c->stream);
         for (long s = 0; s < launches; s += per) {
             int n = (int)std::min(per, launches - s);
@@ -526,6 +618,7 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
     const bool to_array_direct = c->output_type == SODECL_OUTPUT_ARRAY && single;
     if (!to_array_direct && !ensure_stage(c, rows * slab)) return 0;
 
+    if (single && host_orbits(c) > 0) return run_shared(c, launches, cfg, sink, to_array_direct);
     hipEventRecord(c->ev_start, c->stream);
     if (single) {
         if (!enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, c->d_traj[0], c->d_newton, 0,
@@ -813,6 +906,26 @@ int launch_config(const sodecl_ctx* c, int* schedule, int* group, int* block)
 
 }  // namespace sodecl_amd
 
+namespace sodecl_amd {
+
+// Loads the host-share library and keeps a host copy of the parameters of
+// the orbits it will integrate (taken from the device buffer).
+int setup_host_share(sodecl_ctx* c)
+{
+    std::string err;
+    if (!load_host_share(c, err)) return fail(c, err);
+    if (c->hs_p.size() != (size_t)c->orbits * c->npar) {
+        c->hs_p.assign((size_t)c->orbits * c->npar, 0.0);
+        if (c->npar > 0 &&
+            !hip_ok(c, hipMemcpy(c->hs_p.data(), c->d_params, c->hs_p.size() * sizeof(double), hipMemcpyDeviceToHost),
+                    "hipMemcpy(params)"))
+            return 0;
+    }
+    return 1;
+}
+
+}  // namespace sodecl_amd
+
 extern "C" {
 
 void sodecl_default_options(sodecl_options* o)
@@ -860,6 +973,7 @@ void sodecl_destroy(sodecl_ctx* c)
         hipDeviceSynchronize();
     }
     free_device(c);
+    unload_host_share(c);
     delete c;
 }
 
@@ -932,6 +1046,7 @@ int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
     if (o->pairwise < -1 || o->pairwise > 4) return fail(c, "pairwise must be in -1..4");
     if (o->library_sin < 0 || o->library_sin > 1) return fail(c, "library_sin must be 0 or 1");
     if (o->library_exp < 0 || o->library_exp > 1) return fail(c, "library_exp must be 0 or 1");
+    if (!(o->host_share >= 0 && o->host_share < 1)) return fail(c, "host_share must be in [0, 1)");
     // options that change the program text
     bool recompile = o->newton_max_iter != c->opts.newton_max_iter || o->fp_contract != c->opts.fp_contract ||
                      o->pairwise != c->opts.pairwise || o->newton_tol != c->opts.newton_tol ||
@@ -947,7 +1062,9 @@ int sodecl_set_options(sodecl_ctx* c, const sodecl_options* o)
         for (auto& kv : c->programs)
             if (kv.second->module) hipModuleUnload(kv.second->module);
         c->programs.clear();
+        unload_host_share(c);   // its program text follows the same options
     }
+    if (c->ready && c->opts.host_share > 0 && !setup_host_share(c)) return 0;
     return 1;
 }
 
@@ -1007,6 +1124,8 @@ int sodecl_setup(sodecl_ctx* c)
     LaunchCfg cfg = choose_config(c);
     if (!get_program(c, cfg.schedule, cfg.group, cfg.block)) return 0;
     if (!hip_ok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return 0;
+    c->hs_p.clear();
+    if (c->opts.host_share > 0 && !setup_host_share(c)) return 0;
     c->ready = true;
     return 1;
 }
@@ -1019,6 +1138,8 @@ int sodecl_run(sodecl_ctx* c)
     const long launches = num_launches(c->tspan, c->dt, c->ksteps);
     if (launches < 0) return fail(c, "too many kernel launches");
     auto t0 = std::chrono::steady_clock::now();
+    c->last_host_ms = 0.0;
+    c->last_host_orbits = 0;
     int ok;
     if (c->opts.schedule == SODECL_SCHEDULE_PLAIN)
         ok = run_plain(c, launches);
@@ -1060,6 +1181,11 @@ int sodecl_set_output_buffer(sodecl_ctx* c, double* host, size_t capacity)
 
 double sodecl_last_runtime(const sodecl_ctx* c) { return c ? c->last_runtime : 0.0; }
 double sodecl_last_kernel_ms(const sodecl_ctx* c) { return c ? c->last_kernel_ms : 0.0; }
+double sodecl_last_host_ms(const sodecl_ctx* c, int* host_orbits)
+{
+    if (host_orbits) *host_orbits = c ? c->last_host_orbits : 0;
+    return c ? c->last_host_ms : 0.0;
+}
 
 int sodecl_get_launch_config(const sodecl_ctx* c, int* schedule, int* group, int* block)
 {

@@ -27,6 +27,7 @@ struct PluginFeatures {
     bool team = false;              // sode_system_team(...) (team schedule)
     bool jacobian_params = false;   // calc_jacobian_params(y, jp[neq][npar], p) (implicit adjoint)
     bool jacobian_exact = false;    // calc_jacobian_exact(y, jac[neq][neq], p) (implicit adjoint)
+    bool calls_exp = false;         // the source calls exp( (the own exponential's table goes to LDS)
 };
 
 PluginFeatures detect_features(const std::string& source);
@@ -56,6 +57,9 @@ std::string dt_literal(double dt);
 // Program text handed to hipRTC: macro prelude, OpenCL-C compatibility,
 // the user system source, the integrator templates.
 std::string assemble_program(const std::string& system_source, const ProgramKey& key, const PluginFeatures& f);
+// Program of the host share: the same system source and lane integrator as
+// host C++ (sodecl_host_lane over an orbit range, OpenMP across orbits).
+std::string assemble_host_program(const std::string& system_source, const ProgramKey& key, const PluginFeatures& f);
 // Program of the batched implicit integrator: the plugin and the per-class
 // solver once per system size in class_neq, then the persistent kernel.
 std::string assemble_batch_program(const std::string& system_source, const ProgramKey& key, const PluginFeatures& f,
@@ -85,6 +89,11 @@ inline int batch_lanes_per_orbit(int neq) { return neq <= 128 ? 16 : 64; }
 long num_launches(double tspan, double dt, int ksteps);
 
 }  // namespace sodecl_amd
+
+// sodecl_host_lane of a host-share library (integrators.hip, SODECL_HOST_BUILD)
+typedef void (*host_lane_fn)(double* t_state, double* y_state, const double* params, double* counters, double* traj,
+                             long long* newton, int orbits, int first, int last, int s0, int nseg, int final_snap,
+                             long long gstep0, int nthreads);
 
 struct sodecl_ctx {
     // configuration (sodeclmgr ctor arguments, sodeclmgr.cpp:26-83)
@@ -144,6 +153,15 @@ struct sodecl_ctx {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     std::string graph_key;
+
+    // host share (sodecl_options.host_share): the host build of the lane
+    // integrator, and host copies of the state of the orbits it runs
+    void* host_handle = nullptr;
+    host_lane_fn host_fn = nullptr;
+    std::vector<double> hs_t, hs_y, hs_p, hs_cnt;
+    std::vector<long long> hs_newton;
+    double last_host_ms = 0.0;
+    int last_host_orbits = 0;
 };
 
 namespace sodecl_amd {
@@ -169,6 +187,11 @@ int adjoint_into_workspace(sodecl_ctx* c, const double* ckpt_t, const double* ck
 double* scratch_address(int device, std::string& err);
 // Doubles in the static scratch array.
 size_t scratch_doubles();
+// Host share: builds (or finds cached) the host library of the context's
+// program; loads it into the context; unloads it.
+bool build_host_library(const sodecl_ctx* c, std::string& path, std::string& err);
+bool load_host_share(sodecl_ctx* c, std::string& err);
+void unload_host_share(sodecl_ctx* c);
 // stage_ws: when not null, the group kernel also writes every step's stage
 // vectors there (the adjoint's whole-run workspace; group schedule only).
 int run_device_impl(sodecl_ctx* c, double* t0, double* y0, const double* params, double* counters, double* traj,

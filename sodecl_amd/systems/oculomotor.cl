@@ -45,9 +45,18 @@
  * Tn and the burst curve's -|x|). */
 double ocu_div(double a, double d, double rd)
 {
+#if defined(SODECL_STRICT) && !SODECL_STRICT
+    /* the GPU's default arithmetic (contraction on): the product with the
+     * reciprocal, within an ulp of the quotient -- the size of rounding
+     * change contraction makes anyway; right for infinities, NaN and
+     * signed zeros.  Strict arithmetic, and every host build, keep the
+     * quotient's bits. */
+    return a * rd;
+#else
     const double q0 = a * rd;
     const double r = fmax(fma(q0, d, -a), -0x1.fffffffffffffp+1023);
     return fma(-r, rd, q0);
+#endif
 }
 
 /* Burst-neuron firing curve: saturating exponential for x > 0, a scaled

@@ -290,3 +290,71 @@ def test_sampler_rejects_bad_tables_before_launch():
         si.sample_device([(7, 0.0, 1.0)], 10, 1, 1 << 20, 0)    # unknown kind
     with pytest.raises(RuntimeError):
         si.sample_device([], 10, 1, 1 << 20, 0)                 # no columns
+
+
+def _host_share_library(system, solver, dt, tspan, ksteps, neq, npar, nnoi, orbits):
+    lib = ctypes.CDLL(sodecl_amd.library_path())
+    lib.sodecl_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double] + [ctypes.c_int] * 6
+    lib.sodecl_host_library.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.sodecl_host_library.restype = ctypes.c_size_t
+    lib.sodecl_last_error.argtypes = [ctypes.c_void_p]
+    lib.sodecl_last_error.restype = ctypes.c_char_p
+    lib.sodecl_destroy.argtypes = [ctypes.c_void_p]
+    ctx = ctypes.c_void_p()
+    assert lib.sodecl_create(ctypes.byref(ctx), b"kernels", sodecl_amd.system_path(system).encode(), solver, dt, tspan,
+                             ksteps, neq, npar, nnoi, orbits, 1) == 1
+    buf = ctypes.create_string_buffer(4096)
+    n = lib.sodecl_host_library(ctx, buf, len(buf))
+    err = lib.sodecl_last_error(ctx)
+    lib.sodecl_destroy(ctx)
+    assert n > 0, err
+    host = ctypes.CDLL(buf.value.decode())
+    f = host.sodecl_host_lane
+    f.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int] * 6 + [ctypes.c_longlong, ctypes.c_int]
+    return f
+
+
+@pytest.mark.parametrize("solver,name", [(2, "rk4"), (1, "euler"), (3, "ie"), (4, "im"), (0, "em")])
+def test_host_share_library_matches_oracle(solver, name):
+    """The host share's library (the product's lane integrator and the
+    system source compiled for the host, built here through the C ABI)
+    integrates an orbit range of host arrays: within 1e-10 of the oracle,
+    Threefry counters and Newton counts exact, orbits outside the range
+    untouched."""
+    from oracle.oracle import Oracle
+    from sodecl_amd import configs
+    from tests.parity import assert_traj_close
+    orbits, first = 40, 13
+    explicit = solver in (0, 1, 2)
+    dt, tspan, ks = (2e-6, 0.01, 100) if explicit else (1e-5, 0.02, 50)
+    cfg = configs.oculomotor_rk4(orbits=orbits, seed=5, dt=dt, tspan=tspan, ksteps=ks)
+    cfg["y0"][:, 5] += 1.0
+    npar, nnoi = 6, 0
+    params = np.ascontiguousarray(cfg["params"])
+    counters = None
+    if solver == 0:
+        nnoi, npar = 2, 8
+        rng = np.random.default_rng(5)
+        params = np.ascontiguousarray(np.concatenate([params, rng.uniform(0, 0.2, (orbits, 2))], 1))
+        counters = rng.integers(0, 10 ** 11, orbits).astype(np.float64)
+    f = _host_share_library("oculomotor", solver, dt, tspan, ks, 6, npar, nnoi, orbits)
+    launches = oracle_num_launches(tspan, dt, ks)
+    t = np.zeros(orbits)
+    y = np.ascontiguousarray(cfg["y0"]).copy()
+    cnt = None if counters is None else counters.copy()
+    out = np.full((launches + 1, 6, orbits), np.nan)
+    nu = np.zeros(orbits, dtype=np.int64)
+    f(t.ctypes.data, y.ctypes.data, params.ctypes.data, None if cnt is None else cnt.ctypes.data, out.ctypes.data,
+      nu.ctypes.data, orbits, first, orbits, 0, launches, 1, 0, 4)
+    ref = Oracle("oculomotor").run(name, cfg["y0"], params, dt, tspan, ks, counters=counters, nnoi=nnoi)
+    assert_traj_close(out[:, :, first:], ref["out"][:, :, first:], what=f"host share {name}")
+    assert np.isnan(out[:, :, :first]).all()
+    np.testing.assert_array_equal(y[:first], cfg["y0"][:first])
+    assert_traj_close(y[first:].T[None], ref["y"][first:].T[None], what="final state")
+    np.testing.assert_array_equal(t[first:], ref["t"][first:])
+    if solver == 0:
+        np.testing.assert_array_equal(cnt[first:], ref["counters"][first:])
+        np.testing.assert_array_equal(cnt[:first], counters[:first])
+    if not explicit:
+        np.testing.assert_array_equal(nu[first:], ref["newton_updates"][first:])

@@ -272,3 +272,52 @@ def test_team_schedule_device_run_with_checkpoints_matches_host_run():
         if (j * every) % 100 == 0:
             assert np.array_equal(cy[j], tr[(j * every) // 100])
     assert np.allclose(ck_t.cpu().numpy()[3], 3 * every * cfg["dt"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("solver", [2, 3, 0])
+def test_host_share_run_vs_oracle(solver):
+    """sodecl_run with a host share: the last 30% of the orbits integrated on
+    the host cores (the product's lane integrator compiled for the host)
+    while the GPU integrates the rest.  Every orbit within 1e-10 of the
+    oracle, the GPU's orbits bit-identical to a GPU-only run, Threefry
+    counters exact, and a second run continues from the split state."""
+    from sodecl_amd import sodecl_interface as si
+    orbits = 200
+    explicit = solver in (0, 2)
+    dt, tspan, ks = (2e-6, 0.01, 100) if explicit else (1e-5, 0.02, 50)
+    cfg = configs.oculomotor_rk4(orbits=orbits, seed=7, dt=dt, tspan=tspan, ksteps=ks)
+    cfg["y0"][:, 5] += 1.0
+    npar, nnoi, counters = 6, 0, None
+    params = cfg["params"]
+    if solver == 0:
+        rng = np.random.default_rng(7)
+        npar, nnoi = 8, 2
+        params = np.concatenate([params, rng.uniform(0, 0.2, (orbits, 2))], 1)
+        counters = rng.integers(0, 10 ** 11, orbits).astype(np.float64)
+
+    def session(**opts):
+        s = si.Session(cfg["system"], solver, orbits, 6, npar, nnoi, dt, tspan, ks, 0, dict(schedule="lane", **opts))
+        s.setup(cfg["y0"], params, None, counters)
+        return s
+
+    gpu = session()
+    a = gpu.run().reshape(gpu.nsnap, 6, orbits)
+    mix = session(host_share=0.3)
+    b = mix.run().reshape(mix.nsnap, 6, orbits)
+    ms, n_host = mix.host_stats()
+    assert n_host == 60 and ms > 0
+    n_gpu = orbits - n_host
+    np.testing.assert_array_equal(b[:, :, :n_gpu], a[:, :, :n_gpu])
+    ref = Oracle("oculomotor").run({0: "em", 2: "rk4", 3: "ie"}[solver], cfg["y0"], params, dt, tspan, ks,
+                                   counters=counters, nnoi=nnoi)
+    assert_traj_close(b, ref["out"], what="host share")
+    if solver == 0:
+        np.testing.assert_array_equal(mix.counters(), ref["counters"])
+    if solver == 3:
+        np.testing.assert_array_equal(mix.newton_updates(), ref["newton_updates"])
+    # the second run starts from where both shares ended
+    b2 = mix.run().reshape(mix.nsnap, 6, orbits)
+    np.testing.assert_array_equal(b2[0], b[-1])
+    a2 = gpu.run().reshape(gpu.nsnap, 6, orbits)
+    np.testing.assert_array_equal(b2[:, :, :n_gpu], a2[:, :, :n_gpu])
+    assert_traj_close(b2[:, :, n_gpu:], a2[:, :, n_gpu:], what="continued host share")
