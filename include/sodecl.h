@@ -80,9 +80,11 @@ typedef struct sodecl_options {
                                 launch sequence once as a hipGraph and replay it */
     int autotune;            /* with schedule AUTO: time the candidate schedules, group sizes and block
                                 sizes on one segment at setup and keep the fastest (cached on disk) */
-    int fp_contract;         /* 0 = fused multiply-add contraction of the system source's and the solvers'
-                                operations allowed (default; the device library's functions are compiled
-                                as written), 1 = off */
+    int fp_contract;         /* 0 = default arithmetic: fused multiply-add contraction of the system
+                                source's and the solvers' operations, and their divisions as products
+                                with reciprocals (each within about an ulp of the IEEE quotient; the
+                                device library's functions are compiled as written), 1 = strict
+                                arithmetic: neither */
     int verbose;             /* print "Compute results runtime" like the reference */
     size_t traj_budget_bytes;/* device trajectory buffer budget; 0 = 8 GiB */
     int pairwise;            /* group schedule with a plugin that defines the pairwise drift (sode_pair,

@@ -247,12 +247,15 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     }
 }
 
-// With several size classes in one kernel each class's routine is a
-// function of its own (not inlined), so that registers are allocated per
-// class rather than across all of them (measured: no spills instead of 55
-// spilled VGPRs for 32/128/512-equation classes, 2-3% faster).
+// Each class's routine is inlined into the kernel.  As functions of their
+// own (noinline) the classes had registers allocated per class, but hipRTC
+// gives a called function only half of the register budget as VGPRs and
+// moves values through AGPRs (50 AGPR reads per Newton iteration of the
+// 512-equation class); inlined, the 128- and 512-equation Newton loops have
+// 10% fewer VALU instructions, spills stay outside them, and the mixed batch
+// runs 1-2% faster.
 #ifndef SODECL_BATCH_ITEM_ATTR
-#define SODECL_BATCH_ITEM_ATTR
+#define SODECL_BATCH_ITEM_ATTR __attribute__((always_inline))
 #endif
 // Integrates the orbits of one work item -- SODECL_OPW orbits (1, 2 or 4),
 // item[k] on the k-th group of SODECL_LPO lanes (item[k] < 0: none) -- for steps[o]
@@ -320,7 +323,9 @@ __device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__
 #pragma unroll
         for (int c = 0; c < SODECL_BR; ++c) gg[c] = y[c];
         for (int it = 0; it < SODECL_NEWTON_MAXIT; ++it) {
-            __syncthreads();   // previous readers of sh_g are done
+            // The orbit's lanes sit in this wavefront (SODECL_BG is one
+            // wavefront): wavefront-scope ordering of the LDS accesses suffices.
+            sodecl_wave_sync();   // previous readers of sh_g are done
 #pragma unroll
             for (int c = 0; c < SODECL_BR; ++c) {
                 const int comp = lane + c * SODECL_LPO;
@@ -330,7 +335,7 @@ __device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__
                 if (SODECL_OWNS(comp)) sh_g[comp] = (y[c] + gg[c]) * 0.5;
 #endif
             }
-            __syncthreads();
+            sodecl_wave_sync();
             double r[SODECL_BR];
             double err = 0.0;
 #pragma unroll
@@ -367,9 +372,9 @@ __device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__
                     sh_r[SODECL_TI(comp)] = r[c];
                 }
             }
-            __syncthreads();
+            sodecl_wave_sync();
             sodecl_tri_solve(sh_lo, sh_di, sh_up, sh_r, sh_r, lane);
-            __syncthreads();
+            sodecl_wave_sync();
             if (!done) {
 #pragma unroll
                 for (int c = 0; c < SODECL_BR; ++c) {

@@ -12,8 +12,12 @@
 // out[ooff[o] ...], steps[o] / ksteps + 1 of them; params[orbits][npar];
 // newton[o] receives the Newton updates taken.
 
+static_assert(SODECL_BG == 64, "one wavefront per block: the items synchronise at wavefront scope");
+
 // At least two wavefronts per SIMD (a register budget of 256 VGPRs) for these
-// latency-bound wavefronts.
+// latency-bound wavefronts.  (At three, hipRTC gives the per-class functions
+// half of 168 registers as VGPRs and moves values through AGPRs: 16% more
+// VALU instructions in the 512-equation Newton loop, no faster.)
 extern "C" __global__ void __launch_bounds__(SODECL_BG) __attribute__((amdgpu_waves_per_eu(2)))
 sodecl_batch(const int* __restrict__ items, const int* __restrict__ cls, const int* __restrict__ steps,
              const long long* __restrict__ yoff, const long long* __restrict__ ooff, const double* __restrict__ y0,
@@ -33,7 +37,7 @@ sodecl_batch(const int* __restrict__ items, const int* __restrict__ cls, const i
         default:
             break;
         }
-        __syncthreads();   // the next item reuses the LDS rows
+        sodecl_wave_sync();   // the next item reuses the LDS rows
     }
 }
 

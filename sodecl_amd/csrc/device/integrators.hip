@@ -353,7 +353,9 @@ extern "C" void sodecl_host_lane(double* t_state, double* y_state, const double*
                                  double* traj, long long* newton, int orbits, int first, int last, int s0, int nseg,
                                  int final_snap, long long gstep0, int nthreads)
 {
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    // one orbit per work item: an orbit is a whole run of steps, and a share
+    // of a few orbits per thread must still spread over every thread
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
     for (int i = first; i < last; ++i)
         sodecl_lane_orbit(i, t_state, y_state, params, counters, traj, newton, orbits, s0, nseg, final_snap, nullptr,
                           nullptr, 1, gstep0);
@@ -462,16 +464,9 @@ sodecl_team(double* __restrict__ t_state, double* __restrict__ y_state, const do
 #define SODECL_CPL ((_numeq_ + SODECL_G - 1) / SODECL_G)
 
 // G is a power of two no larger than the wavefront, so the lanes of a group
-// always sit in one wavefront.  Their exchanges through LDS need only
-// wavefront-scope ordering: DS instructions of one wavefront execute in
-// order, and the fences plus the wave barrier keep the compiler from moving
-// LDS accesses across this point.  Waves of a block run independently.
-SODECL_INLINE void sodecl_group_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// always sit in one wavefront and their exchanges through LDS need only
+// wavefront-scope ordering.  Waves of a block run independently.
+SODECL_INLINE void sodecl_group_sync() { sodecl_wave_sync(); }
 #define SODECL_OPB (SODECL_BLOCK / SODECL_G)
 #define SODECL_XSTRIDE (_numeq_ | 1)
 #define SODECL_NSTAGE (SODECL_SOLVER == 2 ? 4 : 1)

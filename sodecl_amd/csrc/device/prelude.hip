@@ -20,6 +20,18 @@ __device__ inline double sodecl_dpp(double v)
 #define SODECL_DPP_ROW_SHR(n) (0x110 + (n))   // lane i <- lane i - n within its 16-lane row (0 past the start)
 #define SODECL_DPP_ROW_SHL(n) (0x100 + (n))   // lane i <- lane i + n within its 16-lane row (0 past the end)
 
+// Orders this wavefront's LDS accesses before the call against those after
+// it, for exchanges among the lanes of one wavefront: DS instructions of one
+// wavefront execute in order, and the fences plus the wave barrier keep the
+// compiler from moving LDS accesses across this point.  No other wavefront
+// is waited for.
+__device__ inline void sodecl_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Under the group schedule SODECL_GROUP consecutive lanes of a wavefront
 // share one orbit.  A plugin entry point that the whole group calls together
 // (sode_system_vjp_group) may use:

@@ -120,14 +120,22 @@ std::string epsilon_define(const ProgramKey& k)
 // and sodecl_exp (device/exp.hip) unless the caller asked for the device
 // library's; the solver templates after the system source keep the library's
 // functions.
-std::string system_sin_begin(const ProgramKey& k)
+// Text around the user's system source: the own sine and exponential in
+// place of the library's, and in the default arithmetic (fp_contract = 0)
+// divisions as products with reciprocals (a division by a value fixed over
+// the integration then costs one multiply per evaluation, its reciprocal
+// hoisted out of the step loop; each quotient within about an ulp of the
+// correctly rounded one, the same size of rounding change as contraction).
+std::string system_begin(const ProgramKey& k)
 {
     return std::string(k.library_sin ? "" : "#define sin sodecl_sin\n") +
-           (k.library_exp ? "" : "#define exp sodecl_exp\n");
+           (k.library_exp ? "" : "#define exp sodecl_exp\n") +
+           (k.fp_contract ? "" : "#pragma clang fp reciprocal(on)\n");
 }
-std::string system_sin_end(const ProgramKey& k)
+std::string system_end(const ProgramKey& k)
 {
-    return std::string(k.library_sin ? "" : "#undef sin\n") + (k.library_exp ? "" : "#undef exp\n");
+    return std::string(k.fp_contract ? "" : "#pragma clang fp reciprocal(off)\n") +
+           (k.library_sin ? "" : "#undef sin\n") + (k.library_exp ? "" : "#undef exp\n");
 }
 
 std::string assemble_program(const std::string& system_source, const ProgramKey& k, const PluginFeatures& f)
@@ -173,10 +181,10 @@ std::string assemble_program(const std::string& system_source, const ProgramKey&
     o << kPreludeSource << "\n";
     // The plugin's functions become host+device functions.
     o << "#pragma clang force_cuda_host_device begin\n";
-    o << system_sin_begin(k);
+    o << system_begin(k);
     o << "#line 1 \"system.cl\"\n";
     o << system_source << "\n";
-    o << system_sin_end(k);
+    o << system_end(k);
     o << "#pragma clang force_cuda_host_device end\n";
     o << "#line 1 \"integrators.hip\"\n";
     o << kIntegratorsSource << "\n";
@@ -216,10 +224,10 @@ std::string assemble_host_program(const std::string& system_source, const Progra
          "typedef unsigned char uchar;\n";
     o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
     o << kHostMathSource << "\n";
-    o << system_sin_begin(k);
+    o << system_begin(k);
     o << "#line 1 \"system.cl\"\n";
     o << system_source << "\n";
-    o << system_sin_end(k);
+    o << system_end(k);
     o << "#line 1 \"integrators.hip\"\n";
     o << kHostLaneSource << "\n";
     return o.str();
@@ -253,14 +261,13 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
          "typedef unsigned char uchar;\n";
     o << "#ifndef M_PI\n#define M_PI 3.14159265358979323846\n#endif\n";
     o << kPreludeSource << "\n";
-    if (class_neq.size() > 1) o << "#define SODECL_BATCH_ITEM_ATTR __attribute__((noinline))\n";
     // one namespace per size class: the plugin and the per-class solver
     for (size_t c = 0; c < class_neq.size(); ++c) {
         o << "namespace sodecl_cls" << c << " {\n";
         o << "#define _numeq_ " << class_neq[c] << "\n";
         o << "#define SODECL_LPO " << batch_lanes_per_orbit(class_neq[c]) << "\n";
         o << "#pragma clang force_cuda_host_device begin\n";
-        o << system_sin_begin(k) << "#line 1 \"system.cl\"\n" << system_source << "\n" << system_sin_end(k);
+        o << system_begin(k) << "#line 1 \"system.cl\"\n" << system_source << "\n" << system_end(k);
         o << "#pragma clang force_cuda_host_device end\n";
         o << kBatchClassSource << "\n";
         o << "#undef _numeq_\n#undef SODECL_LPO\n";
