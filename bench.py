@@ -53,12 +53,25 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched implicit measurement")
     ap.add_argument("--no-sampling", action="store_true", help="skip the on-device parameter sampling measurement")
+    ap.add_argument("--no-host-return", action="store_true",
+                    help="skip the measurement of sodecl_run returning the trajectory to host memory")
+    ap.add_argument("--no-host-share", action="store_true",
+                    help="skip the measurement of sodecl_run with and without a host share")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the Kuramoto Euler (1024 orbits) and Euler-Maruyama (262144 orbits) measurements")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run the ahead-of-time streaming copy kernel once over this many MiB "
                          "(reference point for HBM byte counters)")
     return ap.parse_args()
+
+
+# The committed rocprofv3 summary that states each workload's HBM traffic.
+TRAFFIC_PROFILES = {
+    "oculomotor_rk4": "oculomotor_rk4_hbm_traffic.md",
+    "oculomotor_rk4_plain_plugin": "sodecl_lane_oculomotor_rk4_plain.md",
+    "kuramoto_euler": "sodecl_group_kuramoto_euler_n10.md",
+    "kuramoto_em": "sodecl_group_kuramoto_em_n32.md",
+}
 
 
 def traffic_from_profiles(name):
@@ -68,15 +81,17 @@ def traffic_from_profiles(name):
     WRITE_SIZE, scaled by a copy kernel of known size).  This run does not
     measure them: returns (bytes, source) with source naming the file, or
     (None, None)."""
-    rel = os.path.join("profiles", f"{name}_hbm_traffic.md")
+    if name not in TRAFFIC_PROFILES:
+        return None, None
+    rel = os.path.join("profiles", TRAFFIC_PROFILES[name])
     path = os.path.join(HERE, rel)
     if os.path.exists(path):
         with open(path) as f:
             for line in f:
                 if line.strip().startswith("traffic_bytes_per_launch:"):
                     return (float(line.split(":", 1)[1].strip()),
-                            f"{rel}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same command, "
-                            f"not measured by this run")
+                            f"{rel}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same kernel on this "
+                            f"workload (commands in the file), not measured by this run")
     return None, None
 
 
@@ -366,9 +381,91 @@ def batch_bench(solver, reps=1):
     return dict(workload=f"brusselator_batch_{'ie' if solver == 3 else 'im'}", orbits=cfg["orbits"],
                 classes_neq=cfg["class_neq"], steps_total=total, kernel_ms=kms, value=total / (kms * 1e-3),
                 unit="orbit-steps/s", newton_updates=int(updates.sum()),
+                state_updates=int(np.sum(neq * updates)), state_steps=int(np.sum(neq * cfg["steps"])),
                 mean_updates_per_step=float(updates.sum() / total), achieved_tflops=flops / (kms * 1e-3) / 1e12,
                 frac_fp64_peak=flops / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                 finite=bool(torch.isfinite(out).all()))
+
+
+def host_return_bench(cfg, reps=3):
+    """sodecl_run of the workload with the whole trajectory returned to a
+    fresh host array each run (the reference's contract and the Python
+    wrapper's path): the integration streams chunk by chunk through pinned
+    memory into the caller's array while the next chunk integrates.  Wall
+    time of the run, PCIe and host copies included."""
+    import numpy as np
+    from sodecl_amd import configs, sodecl_interface as si
+    steps = si.num_launches(cfg["tspan"], cfg["dt"], cfg["ksteps"]) * cfg["ksteps"]
+    s = configs.session_for(cfg)
+    s.setup(cfg["y0"], cfg["params"], None, cfg["counters"])
+    out = s.run()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = s.run()
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
+    return dict(workload=cfg["name"], path="sodecl_run, trajectory returned to a new host array",
+                ms=wall * 1e3, value=cfg["orbits"] * steps / wall, unit="orbit-steps/s",
+                trajectory_bytes=int(out.nbytes), trajectory_GBps=out.nbytes / wall / 1e9)
+
+
+def host_share_bench(cfg, reps=3):
+    """sodecl_run of the workload on the GPU alone and with a host share:
+    the last orbits integrated on the host cores by the product's lane
+    integrator compiled for the host, while the GPU integrates the rest.
+    Both runs without trajectory output (the share changes only which
+    orbits the GPU integrates; the trajectory's copy to host memory would
+    add the same PCIe transfer to both).  The share is the balance point of
+    the two measured rates, host orbit-steps/s (from a warmed probe run with
+    four orbits per host thread) over host + GPU, rounded down to whole
+    orbits per host thread (an orbit is a whole run of steps)."""
+    import numpy as np
+    from sodecl_amd import configs, sodecl_interface as si
+    orbits = cfg["orbits"]
+    steps = si.num_launches(cfg["tspan"], cfg["dt"], cfg["ksteps"]) * cfg["ksteps"]
+    threads = len(os.sched_getaffinity(0))
+
+    def session(**opts):
+        s = configs.session_for(cfg, output_type=0, **opts)
+        s.setup(cfg["y0"], cfg["params"], None, cfg["counters"])
+        return s
+
+    def median_wall(s):
+        s.run()
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            s.run()
+            walls.append(time.perf_counter() - t0)
+        return float(np.median(walls)), s.last_kernel_ms()
+
+    gpu = session()
+    t_gpu, k_gpu = median_wall(gpu)
+    del gpu
+    probe = session(host_share=min(0.5, 4.0 * threads / orbits))
+    probe.run()
+    probe.run()
+    p_ms, p_n = probe.host_stats()
+    del probe
+    host_rate = p_n * steps / (p_ms * 1e-3)
+    gpu_rate = orbits * steps / (k_gpu * 1e-3)
+    share = host_rate / (host_rate + gpu_rate)
+    # whole orbits per thread: a thread's orbits run one after another
+    per_thread = int(orbits * share / threads)
+    res = dict(workload=cfg["name"], path="sodecl_run without trajectory output", host_threads=threads,
+               host_orbit_steps_per_s=host_rate, gpu_kernel_orbit_steps_per_s=gpu_rate, balance_share=share,
+               gpu_only_ms=t_gpu * 1e3, gpu_only_kernel_ms=k_gpu)
+    if per_thread < 1:
+        res["note"] = "the balance share is below one orbit per host thread: no host share"
+        return res
+    share = (per_thread * threads + 0.5) / orbits
+    mix = session(host_share=share)
+    t_mix, k_mix = median_wall(mix)
+    h_ms, h_n = mix.host_stats()
+    res.update(host_orbits=h_n, shared_ms=t_mix * 1e3, shared_kernel_ms=k_mix, host_ms=h_ms,
+               gain=t_gpu / t_mix - 1.0)
+    return res
 
 
 def main():
@@ -430,6 +527,18 @@ def main():
         log("batched implicit Brusselator (32768 orbits, mixed sizes and step counts)")
         batch = [batch_bench(3), batch_bench(4)]
 
+    host_return = None
+    if not args.no_host_return:
+        log("sodecl_run with the trajectory returned to host memory")
+        host_return = host_return_bench(cfg)
+        if plain is not None:
+            host_return["speedup_vs_plain"] = host_return["value"] / plain["value"]
+
+    share = None
+    if not args.no_host_share and args.config == "oculomotor_rk4":
+        log("host share (sodecl_run on the GPU alone and with the host cores)")
+        share = host_share_bench(cfg)
+
     samp = None
     if not args.no_sampling and args.config == "oculomotor_rk4":
         log("on-device parameter sampling (262144 x 65)")
@@ -481,6 +590,8 @@ def main():
         "fwd_adjoint": adj,
         "batch_implicit": batch,
         "param_sampling": samp,
+        "host_return": host_return,
+        "host_share": share,
         "workloads": legs,
         "speedup_vs_plain": None if plain is None else value / plain["value"],
         "speedup_vs_cpu": None if cpu is None else value / cpu["value"],

@@ -99,10 +99,13 @@ def brusselator_initial(cells, B, A=1.0):
     return y
 
 
-def session_for(cfg, **options):
+def session_for(cfg, output_type=1, **options):
+    """A Session of the workload; output_type 1 = the trajectory returned to
+    host memory (ARRAY), 0 = none."""
     from . import sodecl_interface
     return sodecl_interface.Session(cfg["system"], cfg["solver"], cfg["orbits"], cfg["neq"], cfg["npar"],
-                                    cfg["nnoi"], cfg["dt"], cfg["tspan"], cfg["ksteps"], 0, options)
+                                    cfg["nnoi"], cfg["dt"], cfg["tspan"], cfg["ksteps"], 0, options,
+                                    output_type=output_type)
 
 
 def warm_jit_cache(arch="gfx950"):

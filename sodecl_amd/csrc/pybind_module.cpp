@@ -103,14 +103,15 @@ py::array_t<double> sodeclcall(darr a_t0, darr a_y0, darr a_params, int a_platfo
 class Session {
 public:
     Session(const std::string& system, int solver, int orbits, int neq, int npar, int nnoi, double dt, double tspan,
-            int ksteps, int device, py::dict options, bool system_is_source)
-        : orbits_(orbits), neq_(neq), npar_(npar), nnoi_(nnoi), dt_(dt), tspan_(tspan), ksteps_(ksteps)
+            int ksteps, int device, py::dict options, bool system_is_source, int output_type)
+        : orbits_(orbits), neq_(neq), npar_(npar), nnoi_(nnoi), dt_(dt), tspan_(tspan), ksteps_(ksteps),
+          output_type_(output_type)
     {
         int ok = system_is_source
                      ? sodecl_create_from_source(&ctx_, system.c_str(), solver, dt, tspan, ksteps, neq, npar, nnoi,
-                                                 orbits, SODECL_OUTPUT_ARRAY)
+                                                 orbits, output_type)
                      : sodecl_create(&ctx_, "kernels", system.c_str(), solver, dt, tspan, ksteps, neq, npar, nnoi,
-                                     orbits, SODECL_OUTPUT_ARRAY);
+                                     orbits, output_type);
         if (!ok) throw std::runtime_error(sodecl_last_error(ctx_));
         sodecl_default_options(&opts_);
         apply_options(opts_, options);
@@ -153,8 +154,19 @@ public:
     }
 
     // Host run: returns the flat [snap][eq][orbit] output of this run.
+    // The trajectory [nsnap][neq][orbits] (flat) with ARRAY output; an empty
+    // array with NONE or FILE output.
     py::array_t<double> run()
     {
+        if (output_type_ != SODECL_OUTPUT_ARRAY) {
+            int ok;
+            {
+                py::gil_scoped_release nogil;
+                ok = sodecl_run(ctx_);
+            }
+            if (!ok) fail();
+            return py::array_t<double>(0);
+        }
         const size_t total = (size_t)nsnap() * neq_ * orbits_;
         py::array_t<double> out(total);
         int ok;
@@ -284,6 +296,7 @@ private:
     int orbits_, neq_, npar_, nnoi_;
     double dt_, tspan_;
     int ksteps_;
+    int output_type_ = SODECL_OUTPUT_ARRAY;
     int device_ = 0;
     bool compiled_only_;
     darr y0_, params_, t0_;
@@ -388,10 +401,11 @@ PYBIND11_MODULE(sodecl_interface, m)
         py::arg("dists"), py::arg("rows"), py::arg("seed"), py::arg("out"), py::arg("stream") = 0);
     m.attr("ABI_VERSION") = SODECL_AMD_ABI_VERSION;
     py::class_<Session>(m, "Session")
-        .def(py::init<const std::string&, int, int, int, int, int, double, double, int, int, py::dict, bool>(),
+        .def(py::init<const std::string&, int, int, int, int, int, double, double, int, int, py::dict, bool, int>(),
              py::arg("system"), py::arg("solver"), py::arg("orbits"), py::arg("neq"), py::arg("npar"),
              py::arg("nnoi"), py::arg("dt"), py::arg("tspan"), py::arg("ksteps"), py::arg("device") = 0,
-             py::arg("options") = py::dict(), py::arg("system_is_source") = false)
+             py::arg("options") = py::dict(), py::arg("system_is_source") = false,
+             py::arg("output_type") = (int)SODECL_OUTPUT_ARRAY)
         .def("setup", &Session::setup, py::arg("y0"), py::arg("params"), py::arg("t0") = py::none(),
              py::arg("counters") = py::none())
         .def("run", &Session::run)

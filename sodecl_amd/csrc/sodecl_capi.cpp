@@ -26,6 +26,8 @@
 #include <iostream>
 #include <random>
 #include <sstream>
+#include <sched.h>
+
 #include <thread>
 
 #include "sodecl_internal.hpp"
@@ -413,6 +415,43 @@ int ensure_stage(sodecl_ctx* c, size_t doubles)
     return 1;
 }
 
+// Host threads for host-side work (the host share, copies into the caller's
+// memory): the CPUs this process may run on, at most OMP_NUM_THREADS when
+// that is set (a machine's core count can exceed the share of it a process
+// is given).
+int host_threads()
+{
+    cpu_set_t set;
+    int n = sched_getaffinity(0, sizeof set, &set) == 0 ? CPU_COUNT(&set) : 1;
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+        const int m = std::atoi(e);
+        if (m > 0) n = std::min(n, m);
+    }
+    return std::max(n, 1);
+}
+
+// memcpy split over host threads (slices of at least 2 MiB): a copy from
+// pinned staging memory into a fresh caller array is bound by one core's
+// copy rate and page faults, not by memory bandwidth.
+void parallel_copy(void* dst, const void* src, size_t bytes)
+{
+    const size_t slice_min = (size_t)2 << 20;
+    const int t = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, bytes / slice_min));
+    if (t <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t slice = (bytes / t + 63) & ~(size_t)63;
+    std::vector<std::thread> pool;
+    pool.reserve(t - 1);
+    for (int k = 1; k < t; ++k) {
+        const size_t off = std::min(bytes, slice * k), len = std::min(bytes - off, slice);
+        if (len) pool.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, len); });
+    }
+    std::memcpy(dst, src, std::min(bytes, slice));
+    for (auto& th : pool) th.join();
+}
+
 // Sink for snapshots: host array, file, or nothing.
 struct Sink {
     sodecl_ctx* c;
@@ -443,7 +482,7 @@ struct Sink {
         if (c->output_type == SODECL_OUTPUT_FILE) {
             file.write((const char*)host, (std::streamsize)(n * sizeof(double)));
         } else if (dst) {
-            if (host != dst) std::memcpy(dst, host, n * sizeof(double));
+            if (host != dst) parallel_copy(dst, host, n * sizeof(double));
             dst += n;
         }
     }
@@ -539,7 +578,7 @@ int run_shared(sodecl_ctx* c, long launches, const LaunchCfg& cfg, Sink& sink, b
     hipEventRecord(c->ev_stop, c->stream);
     // the host's orbits while the kernel runs
     const auto h0 = std::chrono::steady_clock::now();
-    const int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int threads = host_threads();
     c->host_fn(c->hs_t.data(), c->hs_y.data(), c->hs_p.data(), c->d_counters ? c->hs_cnt.data() : nullptr, dst,
                c->hs_newton.data(), c->orbits, n_gpu, c->orbits, 0, (int)launches, 1, 0, threads);
     c->last_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
@@ -611,6 +650,14 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
     long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
     size_t budget_rows = std::max<size_t>(2, traj_budget(c) / (slab * sizeof(double)));
     if ((size_t)per + 1 > budget_rows) per = (long)(budget_rows / 2) - 1;
+    // A trajectory of more than 64 MiB for host memory streams in chunks of
+    // about 16 MiB: chunk k integrates while chunk k-1 crosses PCIe into
+    // pinned memory and chunk k-2 is copied into the caller's array by the
+    // host threads (one copy from device into pageable memory measured
+    // about 8.5 GB/s).  A host share needs the single launch.
+    const size_t snap_bytes = slab * sizeof(double);
+    if (host_orbits(c) == 0 && snap_bytes * (size_t)(launches + 1) > ((size_t)64 << 20))
+        per = std::min<long>(per, std::max<long>(1, (long)(((size_t)16 << 20) / snap_bytes)));
     if (per < 1) per = 1;
     const bool single = per >= launches;
     const size_t rows = (size_t)std::min<long>(per, launches) + 1;

@@ -315,6 +315,17 @@ def test_host_share_run_vs_oracle(solver):
         np.testing.assert_array_equal(mix.counters(), ref["counters"])
     if solver == 3:
         np.testing.assert_array_equal(mix.newton_updates(), ref["newton_updates"])
+    # without trajectory output the host share integrates the same orbits
+    if solver in (0, 3):
+        none = si.Session(cfg["system"], solver, orbits, 6, npar, nnoi, dt, tspan, ks, 0,
+                          dict(schedule="lane", host_share=0.3), output_type=0)
+        none.setup(cfg["y0"], params, None, counters)
+        assert none.run().size == 0
+        assert none.host_stats()[1] == n_host
+        if solver == 0:
+            np.testing.assert_array_equal(none.counters(), ref["counters"])
+        else:
+            np.testing.assert_array_equal(none.newton_updates(), ref["newton_updates"])
     # the second run starts from where both shares ended
     b2 = mix.run().reshape(mix.nsnap, 6, orbits)
     np.testing.assert_array_equal(b2[0], b[-1])
