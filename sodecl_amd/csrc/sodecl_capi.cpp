@@ -26,10 +26,8 @@
 #include <iostream>
 #include <random>
 #include <sstream>
-#include <sched.h>
 
-#include <thread>
-
+#include "host_copy.hpp"
 #include "sodecl_internal.hpp"
 
 using sodecl_amd::fail;
@@ -413,43 +411,6 @@ int ensure_stage(sodecl_ctx* c, size_t doubles)
             return 0;
     c->stage_capacity = doubles;
     return 1;
-}
-
-// Host threads for host-side work (the host share, copies into the caller's
-// memory): the CPUs this process may run on, at most OMP_NUM_THREADS when
-// that is set (a machine's core count can exceed the share of it a process
-// is given).
-int host_threads()
-{
-    cpu_set_t set;
-    int n = sched_getaffinity(0, sizeof set, &set) == 0 ? CPU_COUNT(&set) : 1;
-    if (const char* e = std::getenv("OMP_NUM_THREADS")) {
-        const int m = std::atoi(e);
-        if (m > 0) n = std::min(n, m);
-    }
-    return std::max(n, 1);
-}
-
-// memcpy split over host threads (slices of at least 2 MiB): a copy from
-// pinned staging memory into a fresh caller array is bound by one core's
-// copy rate and page faults, not by memory bandwidth.
-void parallel_copy(void* dst, const void* src, size_t bytes)
-{
-    const size_t slice_min = (size_t)2 << 20;
-    const int t = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, bytes / slice_min));
-    if (t <= 1) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t slice = (bytes / t + 63) & ~(size_t)63;
-    std::vector<std::thread> pool;
-    pool.reserve(t - 1);
-    for (int k = 1; k < t; ++k) {
-        const size_t off = std::min(bytes, slice * k), len = std::min(bytes - off, slice);
-        if (len) pool.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, len); });
-    }
-    std::memcpy(dst, src, std::min(bytes, slice));
-    for (auto& th : pool) th.join();
 }
 
 // Sink for snapshots: host array, file, or nothing.
