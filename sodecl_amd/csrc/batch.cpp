@@ -1,6 +1,8 @@
 // C ABI of the batched integrator (include/sodecl.h, no reference
-// counterpart): orbits of several system sizes and individual step counts in
-// one persistent launch (device/batch_class.hip, device/batch_kernel.hip).
+// counterpart): orbits of several system sizes and individual step counts,
+// one kernel per size class with a workgroup per work item, the classes'
+// kernels launched side by side on streams of their own
+// (device/batch_class.hip, device/batch_kernel.hip).
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -24,18 +26,31 @@ struct sodecl_batch {
     int device = 0;
     // per-orbit layout
     int orbits = 0;
-    std::vector<int> cls, steps, order, items;   // items: four orbit slots each (-1: none), longest first
-    int nitems = 0;
+    std::vector<int> cls, steps, order;
     std::vector<long long> yoff, ooff;
     size_t state_size = 0, output_size = 0;
+    // per class: its work items (four orbit slots each, -1: none, longest
+    // first; one workgroup each), its kernel, LDS size and stream; classes
+    // are launched in order of their total cost, the most expensive first
+    struct Class {
+        std::vector<int> items;
+        int nitems = 0;
+        double cost = 0.0;
+        hipFunction_t fn = nullptr;
+        int* d_items = nullptr;
+        size_t lds_bytes = 0;
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Class> classes;
+    std::vector<int> launch_order;
+    hipEvent_t fork = nullptr;
     // device
     std::vector<char> code;
     hipModule_t module = nullptr;
-    hipFunction_t fn = nullptr, fn_reset = nullptr;
-    int *d_items = nullptr, *d_cls = nullptr, *d_steps = nullptr, *d_head = nullptr;
+    int* d_steps = nullptr;
     long long *d_yoff = nullptr, *d_ooff = nullptr;
-    unsigned grid = 0;
-    size_t lds_bytes = 0;
+    bool ready = false;
     std::string last_error, program;
 };
 
@@ -56,17 +71,27 @@ bool bhip(sodecl_batch* b, hipError_t e, const char* what)
 
 void free_batch(sodecl_batch* b)
 {
+    // work queued on the classes' streams ends before their memory goes
+    for (auto& c : b->classes) {
+        if (c.stream) {
+            hipStreamSynchronize(c.stream);
+            hipStreamDestroy(c.stream);
+        }
+        if (c.done) hipEventDestroy(c.done);
+        hipFree(c.d_items);
+    }
+    b->classes.clear();
+    b->launch_order.clear();
+    if (b->fork) hipEventDestroy(b->fork);
+    b->fork = nullptr;
     if (b->module) hipModuleUnload(b->module);
-    hipFree(b->d_items);
-    hipFree(b->d_cls);
     hipFree(b->d_steps);
-    hipFree(b->d_head);
     hipFree(b->d_yoff);
     hipFree(b->d_ooff);
     b->module = nullptr;
-    b->fn = b->fn_reset = nullptr;
-    b->d_items = b->d_cls = b->d_steps = b->d_head = nullptr;
+    b->d_steps = nullptr;
     b->d_yoff = b->d_ooff = nullptr;
+    b->ready = false;
 }
 
 ProgramKey batch_key(const sodecl_batch* b)
@@ -181,30 +206,27 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     b->order.resize(orbits);
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
-    // work items: one orbit, or up to four consecutive orbits of the same
-    // class (similar costs) where that class runs 32 or 16 lanes per orbit
+    // work items of each class: one orbit, or up to four consecutive orbits
+    // (similar costs) where the class runs 32 or 16 lanes per orbit
     const int nclass = (int)b->class_neq.size();
-    std::vector<std::vector<int>> per_class(nclass);
-    for (int o : b->order) per_class[b->cls[o]].push_back(o);
-    struct Item {
-        double cost;
-        int o[4];
-    };
-    std::vector<Item> items;
-    for (int c = 0; c < nclass; ++c) {
-        const int opw = 64 / batch_lanes_per_orbit(b->class_neq[c]);
-        const auto& v = per_class[c];
-        for (size_t i = 0; i < v.size(); i += opw) {
-            Item it{cost[v[i]], {-1, -1, -1, -1}};
-            for (int k = 0; k < opw && i + k < v.size(); ++k) it.o[k] = v[i + k];
-            items.push_back(it);
+    b->classes.assign(nclass, sodecl_batch::Class{});
+    for (int o : b->order) {
+        auto& c = b->classes[b->cls[o]];
+        const int opw = 64 / batch_lanes_per_orbit(b->class_neq[b->cls[o]]);
+        if (c.nitems == 0 || c.items[4 * (c.nitems - 1) + opw - 1] >= 0) {
+            c.items.insert(c.items.end(), {-1, -1, -1, -1});
+            ++c.nitems;
         }
+        int* slot = &c.items[4 * (c.nitems - 1)];
+        int k = 0;
+        while (slot[k] >= 0) ++k;
+        slot[k] = o;
+        c.cost += cost[o];
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) { return a.cost > c.cost; });
-    b->nitems = (int)items.size();
-    b->items.resize(4 * items.size());
-    for (size_t i = 0; i < items.size(); ++i)
-        for (int k = 0; k < 4; ++k) b->items[4 * i + k] = items[i].o[k];
+    b->launch_order.resize(nclass);
+    std::iota(b->launch_order.begin(), b->launch_order.end(), 0);
+    std::stable_sort(b->launch_order.begin(), b->launch_order.end(),
+                     [&](int x, int y) { return b->classes[x].cost > b->classes[y].cost; });
 
     if (!bhip(b, hipSetDevice(b->device), "hipSetDevice")) return 0;
     b->program = assemble_batch_program(b->source, batch_key(b), b->features, b->class_neq);
@@ -212,40 +234,36 @@ int sodecl_batch_setup(sodecl_batch* b, int orbits, const int* orbit_class, cons
     if (!compile_program(b->program, device_arch(b->device), b->opts.fp_contract, b->code, log))
         return bfail(b, "Failed to build program executable:\n" + log);
     if (!bhip(b, hipModuleLoadData(&b->module, b->code.data()), "hipModuleLoadData")) return 0;
-    if (!bhip(b, hipModuleGetFunction(&b->fn, b->module, "sodecl_batch"), "sodecl_batch")) return 0;
-    if (!bhip(b, hipModuleGetFunction(&b->fn_reset, b->module, "sodecl_batch_reset"), "sodecl_batch_reset"))
-        return 0;
-    // LDS per wavefront: per orbit the evaluation point, four padded solve
-    // rows and (more than 8) parameters, times the orbits per wavefront
-    // (batch_class.hip); the largest class sets the launch's size
-    size_t lds_doubles = 1;
-    for (int c = 0; c < nclass; ++c) {
-        const long n = b->class_neq[c];
-        const long lpo = batch_lanes_per_orbit((int)n);
-        const long br = (n + lpo - 1) / lpo;
-        const long tlen = br >= 2 ? (n + 15) / 16 * 16 : n;
-        const long per_orbit = n + 4 * tlen + (b->npar <= 8 ? 0 : b->npar);
-        lds_doubles = std::max(lds_doubles, (size_t)(per_orbit * (64 / lpo)));
-    }
-    b->lds_bytes = lds_doubles * sizeof(double);
-    int per_cu = 0;
-    if (!bhip(b, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, 64, b->lds_bytes),
-              "occupancy"))
-        return 0;
-    int cus = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-    long g = (long)std::max(per_cu, 1) * std::max(cus, 1);
-    b->grid = (unsigned)std::min<long>(g, b->nitems);
     auto up = [&](void** dst, const void* src, size_t bytes, const char* what) {
         return bhip(b, hipMalloc(dst, bytes), what) &&
                bhip(b, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
     };
-    if (!up((void**)&b->d_items, b->items.data(), b->items.size() * sizeof(int), "work items")) return 0;
-    if (!up((void**)&b->d_cls, b->cls.data(), orbits * sizeof(int), "classes")) return 0;
+    for (int ci = 0; ci < nclass; ++ci) {
+        auto& c = b->classes[ci];
+        const std::string name = "sodecl_batch_" + std::to_string(ci);
+        if (!bhip(b, hipModuleGetFunction(&c.fn, b->module, name.c_str()), "class kernel")) return 0;
+        // LDS per wavefront: per orbit four rows of the class's size (the
+        // evaluation point shared with the right-hand side, three band
+        // rows) and (more than 8) parameters, times the orbits per
+        // wavefront (batch_class.hip)
+        const long n = b->class_neq[ci];
+        const long lpo = batch_lanes_per_orbit((int)n);
+        const long br = (n + lpo - 1) / lpo;
+        const long tlen = br >= 2 ? (n + 15) / 16 * 16 : n;
+        const long per_orbit = 4 * tlen + (b->npar <= 8 ? 0 : b->npar);
+        c.lds_bytes = (size_t)(per_orbit * (64 / lpo)) * sizeof(double);
+        if (c.nitems > 0 &&
+            !up((void**)&c.d_items, c.items.data(), c.items.size() * sizeof(int), "work items"))
+            return 0;
+        if (!bhip(b, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate") ||
+            !bhip(b, hipEventCreateWithFlags(&c.done, hipEventDisableTiming), "hipEventCreate"))
+            return 0;
+    }
+    if (!bhip(b, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate")) return 0;
     if (!up((void**)&b->d_steps, b->steps.data(), orbits * sizeof(int), "steps")) return 0;
     if (!up((void**)&b->d_yoff, b->yoff.data(), orbits * sizeof(long long), "state offsets")) return 0;
     if (!up((void**)&b->d_ooff, b->ooff.data(), orbits * sizeof(long long), "output offsets")) return 0;
-    if (!bhip(b, hipMalloc(&b->d_head, sizeof(int)), "queue head")) return 0;
+    b->ready = true;
     return 1;
 }
 
@@ -264,19 +282,30 @@ int sodecl_batch_run_device(sodecl_batch* b, const double* y0, const double* par
                             void* stream)
 {
     if (!b) return 0;
-    if (!b->fn) return bfail(b, "sodecl_batch_setup has not succeeded");
+    if (!b->ready) return bfail(b, "sodecl_batch_setup has not succeeded");
     if (!y0 || !out || !newton || (b->npar > 0 && !params)) return bfail(b, "null device pointer");
     if (!bhip(b, hipSetDevice(b->device), "hipSetDevice")) return 0;
     hipStream_t s = (hipStream_t)stream;
-    void* rargs[] = {&b->d_head};
-    if (!bhip(b, hipModuleLaunchKernel(b->fn_reset, 1, 1, 1, 1, 1, 1, 0, s, rargs, nullptr), "reset")) return 0;
-    int nitems = b->nitems;
-    void* args[] = {&b->d_items, &b->d_cls, &b->d_steps, &b->d_yoff, &b->d_ooff, (void*)&y0, (void*)&params,
-                    &out, &newton, &b->d_head, &nitems};
-    return bhip(b, hipModuleLaunchKernel(b->fn, b->grid, 1, 1, 64, 1, 1, (unsigned)b->lds_bytes, s, args, nullptr),
-                "hipModuleLaunchKernel(batch)")
-               ? 1
-               : 0;
+    // every class's kernel on its own stream after the work queued on the
+    // caller's stream, then the caller's stream after all of them (its waits
+    // are queued last: a class stream that shares a hardware queue with the
+    // caller's would otherwise wait behind them)
+    if (!bhip(b, hipEventRecord(b->fork, s), "hipEventRecord")) return 0;
+    for (int ci : b->launch_order) {
+        auto& c = b->classes[ci];
+        if (c.nitems == 0) continue;
+        void* args[] = {&c.d_items, &b->d_steps, &b->d_yoff, &b->d_ooff, (void*)&y0, (void*)&params, &out, &newton};
+        if (!bhip(b, hipStreamWaitEvent(c.stream, b->fork, 0), "hipStreamWaitEvent") ||
+            !bhip(b, hipModuleLaunchKernel(c.fn, (unsigned)c.nitems, 1, 1, 64, 1, 1, (unsigned)c.lds_bytes, c.stream,
+                                           args, nullptr),
+                  "hipModuleLaunchKernel(batch)") ||
+            !bhip(b, hipEventRecord(c.done, c.stream), "hipEventRecord"))
+            return 0;
+    }
+    for (int ci : b->launch_order)
+        if (b->classes[ci].nitems > 0 && !bhip(b, hipStreamWaitEvent(s, b->classes[ci].done, 0), "hipStreamWaitEvent"))
+            return 0;
+    return 1;
 }
 
 const char* sodecl_batch_last_error(const sodecl_batch* b) { return b ? b->last_error.c_str() : "null batch"; }

@@ -39,10 +39,13 @@
 #define SODECL_TI(i) (i)
 #define SODECL_TLEN (_numeq_)
 #endif
-// LDS doubles per orbit: evaluation point, four padded solve rows, and the
-// parameters when there are more than 8 (fewer stay in registers)
+// LDS doubles per orbit: four rows -- the evaluation point, which the
+// right-hand side of the solve (then its solution) overwrites once the
+// Jacobian has read it, and the three band rows -- and the parameters when
+// there are more than 8 (fewer stay in registers).  16 KiB for 512
+// equations, so that ten wavefronts fit a CU's LDS.
 #define SODECL_PLDS (_numpar_ <= 8 ? 0 : _numpar_)
-#define SODECL_ORBIT_LDS (_numeq_ + 4 * SODECL_TLEN + SODECL_PLDS)
+#define SODECL_ORBIT_LDS (4 * SODECL_TLEN + SODECL_PLDS)
 
 // Coefficients of row i of the reference's tridiagonal matrix.  The band
 // rows are written with the reference's zeros in place (a_i = 0 outside
@@ -247,13 +250,10 @@ SODECL_INLINE void sodecl_tri_solve(const double* lo, const double* di, const do
     }
 }
 
-// Each class's routine is inlined into the kernel.  As functions of their
-// own (noinline) the classes had registers allocated per class, but hipRTC
-// gives a called function only half of the register budget as VGPRs and
-// moves values through AGPRs (50 AGPR reads per Newton iteration of the
-// 512-equation class); inlined, the 128- and 512-equation Newton loops have
-// 10% fewer VALU instructions, spills stay outside them, and the mixed batch
-// runs 1-2% faster.
+// The routine is inlined into its class's kernel (a called function gets
+// only half of the register budget from hipRTC as VGPRs and moves values
+// through AGPRs: 50 AGPR reads per Newton iteration of the 512-equation
+// class, 10% more VALU instructions).
 #ifndef SODECL_BATCH_ITEM_ATTR
 #define SODECL_BATCH_ITEM_ATTR __attribute__((always_inline))
 #endif
@@ -286,7 +286,7 @@ __device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__
     smax = max(smax, max(__builtin_amdgcn_readlane(steps, 2 * SODECL_LPO), __builtin_amdgcn_readlane(steps, 3 * SODECL_LPO)));
 #endif
     double* const sh_g = lds + (size_t)part * SODECL_ORBIT_LDS;   // evaluation point (g, or (y+g)/2), plain layout
-    double* const sh_r = sh_g + _numeq_;           // residual -> right-hand side -> solution (padded layout)
+    double* const sh_r = sh_g;   // then the residual -> right-hand side -> solution (rotated layout)
     double* const sh_lo = sh_r + SODECL_TLEN;
     double* const sh_di = sh_lo + SODECL_TLEN;
     double* const sh_up = sh_di + SODECL_TLEN;
@@ -369,8 +369,13 @@ __device__ SODECL_BATCH_ITEM_ATTR void sodecl_batch_item(const int* __restrict__
                     sh_lo[SODECL_TI(comp)] = comp < _numeq_ - 1 ? lo * scale * _m_dt_ : 0.0;
                     sh_up[SODECL_TI(comp)] = (comp >= 1 && comp < _numeq_ - 2) ? up * scale * _m_dt_ : 0.0;
                     sh_di[SODECL_TI(comp)] = di * scale * _m_dt_ + 1;
-                    sh_r[SODECL_TI(comp)] = r[c];
                 }
+            }
+            sodecl_wave_sync();   // every read of the evaluation point is done
+#pragma unroll
+            for (int c = 0; c < SODECL_BR; ++c) {
+                const int comp = lane + c * SODECL_LPO;
+                if (SODECL_OWNS(comp)) sh_r[SODECL_TI(comp)] = r[c];
             }
             sodecl_wave_sync();
             sodecl_tri_solve(sh_lo, sh_di, sh_up, sh_r, sh_r, lane);

@@ -272,13 +272,14 @@ std::string assemble_batch_program(const std::string& system_source, const Progr
         o << kBatchClassSource << "\n";
         o << "#undef _numeq_\n#undef SODECL_LPO\n";
         o << "}  // namespace sodecl_cls" << c << "\n";
+        // the class's own kernel
+        o << "#define SODECL_BATCH_KERNEL sodecl_batch_" << c << "\n";
+        o << "#define SODECL_BATCH_ITEM sodecl_cls" << c << "::sodecl_batch_item\n";
+        o << "#define SODECL_BATCH_WAVES " << batch_waves_per_simd(class_neq[c]) << "\n";
+        o << kBatchKernelSource << "\n";
+        o << "#undef SODECL_BATCH_KERNEL\n#undef SODECL_BATCH_ITEM\n#undef SODECL_BATCH_WAVES\n";
     }
-    o << "#define SODECL_BATCH_CASES";
-    for (size_t c = 0; c < class_neq.size(); ++c)
-        o << " case " << c << ": sodecl_cls" << c
-          << "::sodecl_batch_item(io, steps, yoff, ooff, y0, params, out, newton, sodecl_lds); break;";
-    o << "\n";
-    o << kBatchKernelSource << "\n";
+
     return o.str();
 }
 

@@ -85,6 +85,17 @@ std::string device_arch(int device);
 // of neq equations: 16 (four orbits per wavefront) up to 32 equations, 32
 // (two) up to 128, else 64.
 inline int batch_lanes_per_orbit(int neq) { return neq <= 128 ? 16 : 64; }
+// Occupancy hint of a size class's kernel (the wavefronts per SIMD its
+// register budget is sized for): up to two rows per lane need few registers
+// (four); a whole wavefront per orbit with up to 8 rows per lane fits 168
+// registers with no spill in its Newton loop (three); four orbits of 16
+// lanes with 8 rows per lane spill at 168 (two).
+inline int batch_waves_per_simd(int neq)
+{
+    const int lpo = batch_lanes_per_orbit(neq), rows = (neq + lpo - 1) / lpo;
+    if (rows <= 2) return 4;
+    return (lpo == 64 && rows <= 8) ? 3 : 2;
+}
 
 long num_launches(double tspan, double dt, int ksteps);
 
