@@ -298,38 +298,49 @@ SODECL_INLINE void sodecl_lane_orbit(int i, double* __restrict__ t_state, double
             SODECL_UNROLL
             for (int m = 0; m < _numeq_; ++m) o[(size_t)m * orbits] = y[m];
         }
-        for (int it = 0; it < _numsteps_; ++it, ++g) {
-            if (ckpt_y && g == ck_step) {
-                const long long k = ck++;
-                ck_step += ckpt_every;
-                SODECL_UNROLL
-                for (int m = 0; m < _numeq_; ++m) ckpt_y[(size_t)k * slab + (size_t)m * orbits + i] = y[m];
-                ckpt_t[(size_t)k * orbits + i] = t;
+        // The segment's steps in runs that end at the next checkpoint, so that
+        // the step loop tests only its count (with one wavefront per SIMD, each
+        // scalar instruction in it takes an issue cycle from the VALU).
+        for (int it = 0; it < _numsteps_;) {
+            int run = _numsteps_ - it;
+            if (ckpt_y) {
+                if (g == ck_step) {
+                    const long long k = ck++;
+                    ck_step += ckpt_every;
+                    SODECL_UNROLL
+                    for (int m = 0; m < _numeq_; ++m) ckpt_y[(size_t)k * slab + (size_t)m * orbits + i] = y[m];
+                    ckpt_t[(size_t)k * orbits + i] = t;
+                }
+                if (ck_step - g < run) run = (int)(ck_step - g);   // g < ck_step here
             }
+            it += run;
+            g += run;
+            for (; run > 0; --run) {
 #if SODECL_IS_SDE
-            // Noise for this step (stochastic_solver_caller.cl:44-66)
-            int nc = 0;
-            for (int in = 0; in < calls; ++in) {
-                double z0, z1;
-                sodecl_noise_pair(i, cnt, dtsqrt, &z0, &z1);
-                cnt = cnt + 1;
-                if (nc < _numnoi_) noise[nc++] = z0;
-                if (nc < _numnoi_) noise[nc++] = z1;
-            }
-            // Euler-Maruyama (stochastic_euler.cl:10-22)
-            {
-                double st[_numeq_];
-                sode_system(t, y, yn, p);
-                sode_system_stoch(t, y, st, p, noise);
-                SODECL_UNROLL
-                for (int m = 0; m < _numeq_; ++m) yn[m] = y[m] + yn[m] * _m_dt_ + st[m];
-            }
+                // Noise for this step (stochastic_solver_caller.cl:44-66)
+                int nc = 0;
+                for (int in = 0; in < calls; ++in) {
+                    double z0, z1;
+                    sodecl_noise_pair(i, cnt, dtsqrt, &z0, &z1);
+                    cnt = cnt + 1;
+                    if (nc < _numnoi_) noise[nc++] = z0;
+                    if (nc < _numnoi_) noise[nc++] = z1;
+                }
+                // Euler-Maruyama (stochastic_euler.cl:10-22)
+                {
+                    double st[_numeq_];
+                    sode_system(t, y, yn, p);
+                    sode_system_stoch(t, y, st, p, noise);
+                    SODECL_UNROLL
+                    for (int m = 0; m < _numeq_; ++m) yn[m] = y[m] + yn[m] * _m_dt_ + st[m];
+                }
 #else
-            nu += sodecl_lane_step(t, y, yn, p);
+                nu += sodecl_lane_step(t, y, yn, p);
 #endif
-            t = t + _m_dt_;
-            SODECL_UNROLL
-            for (int m = 0; m < _numeq_; ++m) y[m] = yn[m];
+                t = t + _m_dt_;
+                SODECL_UNROLL
+                for (int m = 0; m < _numeq_; ++m) y[m] = yn[m];
+            }
         }
     }
     if (traj && final_snap) {
