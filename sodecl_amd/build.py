@@ -21,7 +21,7 @@ ARCH = os.environ.get("SODECL_AMD_ARCH", "gfx950")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 # Every HIP compile: no packed FP32 instructions.
 NO_PK_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-DEVICE_SOURCES = ["random.hip", "integrators.hip", "adjoint.hip"]
+DEVICE_SOURCES = ["random.hip", "gauss.hip", "integrators.hip", "adjoint.hip"]
 HOST_SOURCES = ["jit.cpp", "sodecl_capi.cpp", "sodeclmgr.cpp", "adjoint.cpp", "batch.cpp", "host_share.cpp"]
 
 
@@ -51,7 +51,7 @@ def embed_device_sources():
     out = os.path.join(CSRC, "_embedded_device.inc")
     new = (_raw_string("kPreludeSource", ["sin.hip", "exp.hip", "prelude.hip"]) + _raw_string("kIntegratorsSource", DEVICE_SOURCES) +
            _raw_string("kHostMathSource", ["sin.hip", "exp.hip"]) +
-           _raw_string("kHostLaneSource", ["random.hip", "integrators.hip"]) +
+           _raw_string("kHostLaneSource", ["random.hip", "gauss.hip", "integrators.hip"]) +
            _raw_string("kBatchClassSource", ["batch_class.hip"]) +
            _raw_string("kBatchKernelSource", ["batch_kernel.hip"]))
     if not os.path.exists(out) or open(out).read() != new:

@@ -89,7 +89,9 @@ SODECL_INLINE sodecl_u64 sodecl_counter_word(double c)
 #define SODECL_RNG_KEY0 0ull
 #define SODECL_RNG_KEY1 0xf00dcafeull
 
-// One Box-Muller pair scaled by sqrt(dt) (stochastic_solver_caller.cl:54-65).
+// One Box-Muller pair scaled by sqrt(dt) (stochastic_solver_caller.cl:54-65),
+// with the logarithm, sine and cosine specialised to the uniforms' ranges
+// (gauss.hip).
 SODECL_INLINE void sodecl_noise_pair(int orbit, double counter, double dtsqrt, double* z0, double* z1)
 {
     sodecl_u64 w0, w1;
@@ -97,9 +99,9 @@ SODECL_INLINE void sodecl_noise_pair(int orbit, double counter, double dtsqrt, d
                            &w0, &w1);
     const double u0 = sodecl_u01(w0);
     const double u1 = sodecl_u01(w1);
-    const double r = sqrt(-2 * log(u0));
+    const double r = sqrt(-2 * sodecl_log_positive(u0));
     double s, c;
-    sincos(2 * M_PI * u1, &s, &c);
+    sodecl_sincos_reduced(2 * M_PI * u1, &s, &c);
     *z0 = (r * c) * dtsqrt;
     *z1 = (r * s) * dtsqrt;
 }

@@ -15,7 +15,30 @@ typedef unsigned long long sodecl_u64;
 // rounds.  Identical output to Random123's threefry2x64 (threefry.h:178-293),
 // which the reference's SDE caller uses.
 // ---------------------------------------------------------------------------
-SODECL_INLINE sodecl_u64 sodecl_rotl(sodecl_u64 v, int n) { return (v << n) | (v >> (64 - n)); }
+// v rotated left by n, 0 < n < 64 (a constant at every call).  On the GPU
+// each 32-bit half of the result is one funnel shift of the two input halves
+// (v_alignbit_b32: the low word of {a, b} >> s); the compiler's own lowering
+// of the 64-bit shifts and or takes three to four instructions.
+SODECL_INLINE sodecl_u64 sodecl_rotl(sodecl_u64 v, int n)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+    unsigned rh, rl;
+    if (n == 32) {
+        rh = lo;
+        rl = hi;
+    } else if (n < 32) {
+        rh = __builtin_amdgcn_alignbit(hi, lo, 32 - n);   // (hi << n) | (lo >> (32 - n))
+        rl = __builtin_amdgcn_alignbit(lo, hi, 32 - n);   // (lo << n) | (hi >> (32 - n))
+    } else {
+        rh = __builtin_amdgcn_alignbit(lo, hi, 64 - n);   // the halves swapped, then by n - 32
+        rl = __builtin_amdgcn_alignbit(hi, lo, 64 - n);
+    }
+    return ((sodecl_u64)rh << 32) | rl;
+#else
+    return (v << n) | (v >> (64 - n));
+#endif
+}
 
 #define SODECL_TF_ROUND(R) \
     x0 += x1;              \

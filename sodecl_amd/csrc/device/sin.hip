@@ -51,19 +51,22 @@ SODECL_MATH_FN double sodecl_with_hi_word(double v, unsigned hi)
     return v;
 }
 
-// fma(a, b, c) as one three-operand instruction.  Left to the compiler, a
-// chain step whose addend is a loop-invariant constant often became a copy
-// of the constant followed by the two-operand form (whose addend is also its
-// destination): two instructions instead of one.
+// fma(a, b, k) for a constant addend k, as one three-operand instruction
+// that reads k from a scalar register pair.  Left to the compiler, a chain
+// step whose addend is a constant became the two-operand form, whose addend
+// is also its destination: a copy of the constant into vector registers
+// before every step, or, hoisted out of a loop, two vector registers held
+// for each constant through the whole loop (fewer wavefronts per SIMD).
+// k must be the same in every lane (a literal constant at every call).
 #if defined(__HIP_DEVICE_COMPILE__)
-__device__ inline double sodecl_fma3(double a, double b, double c)
+__device__ inline double sodecl_fma_k(double a, double b, double k)
 {
     double d;
-    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(k));
     return d;
 }
 #else
-#define sodecl_fma3 fma
+#define sodecl_fma_k fma
 #endif
 
 // sin x by the reduction above; *library is set where sodecl_sin returns the
@@ -84,13 +87,13 @@ SODECL_MATH_FN double sodecl_sin_reduced(double x, bool* library)
     *library = !(ax <= 0x1p30) || fabs(r2) < k * (4.0 * p2);
     const double z = r2 * r2;
     double p = -0x1.283dfd82ab975p-57;
-    p = sodecl_fma3(p, z, 0x1.951051ba25afep-49);
-    p = sodecl_fma3(p, z, -0x1.ae7f03ca459a2p-41);
-    p = sodecl_fma3(p, z, 0x1.612460f6bebccp-33);
-    p = sodecl_fma3(p, z, -0x1.ae64567eff356p-26);
-    p = sodecl_fma3(p, z, 0x1.71de3a556c3ecp-19);
-    p = sodecl_fma3(p, z, -0x1.a01a01a01a018p-13);
-    p = sodecl_fma3(p, z, 0x1.1111111111111p-7);
+    p = sodecl_fma_k(p, z, 0x1.951051ba25afep-49);
+    p = sodecl_fma_k(p, z, -0x1.ae7f03ca459a2p-41);
+    p = sodecl_fma_k(p, z, 0x1.612460f6bebccp-33);
+    p = sodecl_fma_k(p, z, -0x1.ae64567eff356p-26);
+    p = sodecl_fma_k(p, z, 0x1.71de3a556c3ecp-19);
+    p = sodecl_fma_k(p, z, -0x1.a01a01a01a018p-13);
+    p = sodecl_fma_k(p, z, 0x1.1111111111111p-7);
     const double zl = fma(r2, r2, -z);       // r2^2 = z + zl
     const double v = z * r2;                 // r2^3 = v + vl
     const double vl = fma(zl, r2, fma(z, r2, -v));
@@ -98,7 +101,7 @@ SODECL_MATH_FN double sodecl_sin_reduced(double x, bool* library)
     const double s1l = -0x1.5555555555555p-57;
     const double u = fma(v, s1, r2);
     const double e = fma(v, s1, r2 - u);     // r2 + v s1 = u + e
-    double w = fma(v, sodecl_fma3(z, p, s1l), e);    // + v (s1l + z p)
+    double w = fma(v, sodecl_fma_k(z, p, s1l), e);    // + v (s1l + z p)
     w = fma(vl, s1, w);
     w = fma(t, fma(z, fma(z, 0x1.5555555555555p-5, -0.5), 1.0), w);   // + t (1 - z/2 + z^2/24)
     const double s = u + w;
