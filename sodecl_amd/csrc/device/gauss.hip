@@ -9,8 +9,9 @@
 // frexp and ldexp) with contraction off, so the file is also valid host
 // C++ (define SODECL_MATH_FN first) and its host build computes the same
 // bits as the GPU; gauss_coefficients.py derives every constant.
-// Measured against mpmath (tests/test_gauss.py): at most 0.52 ulp for the
-// logarithm on [2^-53, 1), 0.51 ulp for the sine and cosine on (0, 2 pi].
+// Measured against x87 long double on 10^7 arguments each: at most 0.66 ulp
+// for the logarithm of the uniforms, 0.58 ulp for the sine and 0.57 ulp for
+// the cosine on (0, 2 pi]; correctly rounded in more than 99% of cases.
 
 #ifndef SODECL_MATH_FN
 #define SODECL_MATH_FN __host__ __device__ inline
