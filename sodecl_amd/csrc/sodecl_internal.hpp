@@ -82,14 +82,15 @@ bool compile_program(const std::string& text, const std::string& arch, int fp_co
                      std::string& log, bool ilp_schedule = false);
 std::string device_arch(int device);
 // Lanes of a wavefront per orbit in the batched integrator for a size class
-// of neq equations: 16 (four orbits per wavefront) up to 32 equations, 32
-// (two) up to 128, else 64.
+// of neq equations: 16 (four orbits per wavefront) up to 128 equations,
+// else 64.
 inline int batch_lanes_per_orbit(int neq) { return neq <= 128 ? 16 : 64; }
 // Occupancy hint of a size class's kernel (the wavefronts per SIMD its
 // register budget is sized for): up to two rows per lane need few registers
 // (four); a whole wavefront per orbit with up to 8 rows per lane fits 168
 // registers with no spill in its Newton loop (three); four orbits of 16
-// lanes with 8 rows per lane spill at 168 (two).
+// lanes with 8 rows per lane keep two (at three they also fit 168 registers,
+// but the launch of all classes side by side took 6-8% longer).
 inline int batch_waves_per_simd(int neq)
 {
     const int lpo = batch_lanes_per_orbit(neq), rows = (neq + lpo - 1) / lpo;
