@@ -789,69 +789,92 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
                 if (comp < _numeq_) traj[(size_t)(s0 + s) * slab + (size_t)comp * orbits + orbit] = y[c];
             }
         }
-        for (int it = 0; it < _numsteps_; ++it, ++g) {
-            if (ckpt_y && g == ck_step) {
-                const long long k = ck++;
-                ck_step += ckpt_every;
-                if (active) {
+        // The segment's steps in runs that end at the next checkpoint, so that
+        // the step loop tests only its count; the stage-vector store is chosen
+        // once per run (a kernel argument, the same for every lane, so the
+        // step's lane exchanges see the whole wavefront either way).  With one
+        // wavefront per SIMD each scalar instruction in the step loop takes an
+        // issue cycle from the VALU.
+        for (int it = 0; it < _numsteps_;) {
+            int run = _numsteps_ - it;
+            if (ckpt_y) {
+                if (g == ck_step) {
+                    const long long k = ck++;
+                    ck_step += ckpt_every;
+                    if (active) {
+#pragma unroll
+                        for (int c = 0; c < SODECL_CPL; ++c) {
+                            const int comp = lane + c * SODECL_G;
+                            if (comp < _numeq_) ckpt_y[(size_t)k * slab + (size_t)comp * orbits + orbit] = y[c];
+                        }
+                        if (lane == 0) ckpt_t[(size_t)k * orbits + orbit] = t;
+                    }
+                }
+                if (ck_step - g < run) run = (int)(ck_step - g);   // g < ck_step here
+            }
+            const int end = it + run;
+#if SODECL_IS_SDE
+            for (; it < end; ++it, ++g) {
+                // Noise: Threefry call k of this step goes to lane k of the group;
+                // the counter of call k is counter + k, as the reference's
+                // sequential +1 per call (stochastic_solver_caller.cl:44-66).
+                // The noise does not depend on the state: a group with at least
+                // twice as many lanes as calls draws the next step of the same
+                // segment in the same pass (lanes calls .. 2 calls - 1, counters
+                // counter + calls + k, into the next step's buffer), and that
+                // step draws none.
+                if (!noise_ready) {
+                    const int n = (SODECL_G >= 2 * SODECL_CALLS && it + 1 < _numsteps_) ? 2 * calls : calls;
+                    for (int k = lane; k < n; k += SODECL_G) {
+                        const int nb = k < calls ? buf : (buf ^ 1);
+                        const int kk = k < calls ? k : k - calls;
+                        double z0, z1;
+                        sodecl_noise_pair(orbit, cnt + (double)k, dtsqrt, &z0, &z1);
+                        if (2 * kk < _numnoi_) sh_n[nb][slot][2 * kk] = z0;
+                        if (2 * kk + 1 < _numnoi_) sh_n[nb][slot][2 * kk + 1] = z1;
+                    }
+                    noise_ready = n > calls;
+                } else {
+                    noise_ready = false;
+                }
+                cnt = cnt + (double)calls;
+                sodecl_publish(sh_x[buf][slot], y, lane);
+                sodecl_group_sync();
+                // Euler-Maruyama (stochastic_euler.cl:10-22)
+                {
+                    double f[SODECL_CPL];
+                    SODECL_EVAL(t, sh_x[buf][slot], f);
 #pragma unroll
                     for (int c = 0; c < SODECL_CPL; ++c) {
                         const int comp = lane + c * SODECL_G;
-                        if (comp < _numeq_) ckpt_y[(size_t)k * slab + (size_t)comp * orbits + orbit] = y[c];
-                    }
-                    if (lane == 0) ckpt_t[(size_t)k * orbits + orbit] = t;
-                }
-            }
-#if SODECL_IS_SDE
-            // Noise: Threefry call k of this step goes to lane k of the group;
-            // the counter of call k is counter + k, as the reference's
-            // sequential +1 per call (stochastic_solver_caller.cl:44-66).
-            // The noise does not depend on the state: a group with at least
-            // twice as many lanes as calls draws the next step of the same
-            // segment in the same pass (lanes calls .. 2 calls - 1, counters
-            // counter + calls + k, into the next step's buffer), and that
-            // step draws none.
-            if (!noise_ready) {
-                const int n = (SODECL_G >= 2 * SODECL_CALLS && it + 1 < _numsteps_) ? 2 * calls : calls;
-                for (int k = lane; k < n; k += SODECL_G) {
-                    const int nb = k < calls ? buf : (buf ^ 1);
-                    const int kk = k < calls ? k : k - calls;
-                    double z0, z1;
-                    sodecl_noise_pair(orbit, cnt + (double)k, dtsqrt, &z0, &z1);
-                    if (2 * kk < _numnoi_) sh_n[nb][slot][2 * kk] = z0;
-                    if (2 * kk + 1 < _numnoi_) sh_n[nb][slot][2 * kk + 1] = z1;
-                }
-                noise_ready = n > calls;
-            } else {
-                noise_ready = false;
-            }
-            cnt = cnt + (double)calls;
-            sodecl_publish(sh_x[buf][slot], y, lane);
-            sodecl_group_sync();
-            // Euler-Maruyama (stochastic_euler.cl:10-22)
-            {
-                double f[SODECL_CPL];
-                SODECL_EVAL(t, sh_x[buf][slot], f);
-#pragma unroll
-                for (int c = 0; c < SODECL_CPL; ++c) {
-                    const int comp = lane + c * SODECL_G;
-                    if (comp < _numeq_) {
-                        const double gn = sode_system_stoch_i(comp, t, sh_x[buf][slot], pp, sh_n[buf][slot]);
-                        y[c] = y[c] + f[c] * _m_dt_ + gn;
+                        if (comp < _numeq_) {
+                            const double gn = sode_system_stoch_i(comp, t, sh_x[buf][slot], pp, sh_n[buf][slot]);
+                            y[c] = y[c] + f[c] * _m_dt_ + gn;
+                        }
                     }
                 }
+                buf ^= 1;
+                t = t + _m_dt_;
             }
-            buf ^= 1;
 #else
-            double* row = nullptr;
-            if (stage_ws && active) {
-                // every step's stage vectors for an adjoint without recomputation
-                row = stage_ws + ((size_t)orbit * (size_t)ws_steps + (size_t)g) * SODECL_WROW;
-                if (lane == 0) row[SODECL_NSTAGE * _numeq_] = t;
+            if (stage_ws) {
+                for (; it < end; ++it, ++g) {
+                    double* row = nullptr;
+                    if (active) {
+                        // every step's stage vectors for an adjoint without recomputation
+                        row = stage_ws + ((size_t)orbit * (size_t)ws_steps + (size_t)g) * SODECL_WROW;
+                        if (lane == 0) row[SODECL_NSTAGE * _numeq_] = t;
+                    }
+                    sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, row);
+                    t = t + _m_dt_;
+                }
+            } else {
+                for (; it < end; ++it, ++g) {
+                    sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, nullptr);
+                    t = t + _m_dt_;
+                }
             }
-            sodecl_group_step(t, y, sh_x, buf, slot, lane, pp, S, row);
 #endif
-            t = t + _m_dt_;
         }
     }
     if (active) {
