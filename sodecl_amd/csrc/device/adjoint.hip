@@ -71,6 +71,7 @@ sodecl_group_adjoint(const double* __restrict__ ckpt_t, const double* __restrict
 #else
     double* const S = nullptr;
 #endif
+    sodecl_pairs_init(S, lane);
     for (int k = lane; k < _numpar_; k += SODECL_G) {
         pp[k] = params[(size_t)ob * _numpar_ + k];
         gp[k] = 0.0;

@@ -577,15 +577,8 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S,
 #define SODECL_NPAIRS (_numeq_ * SODECL_HALF + (SODECL_EVEN ? _numeq_ / 2 : 0))
 SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, double* S, int lane, double (&f)[SODECL_CPL])
 {
-#if SODECL_USE_PAIRS == 1
-#pragma unroll
-    for (int c = 0; c < SODECL_CPL; ++c) {
-        const int i = lane + c * SODECL_G;
-        // pair(i, i) = -pair(i, i) is zero for finite states (and a
-        // non-finite state makes the whole row non-finite): not evaluated
-        if (i < _numeq_) S[i * SODECL_SSTRIDE + i] = 0.0;
-    }
-#endif
+    // (the diagonal of the full matrix is written once per launch, by
+    // sodecl_pairs_init)
     // the pairs go round-robin to all G lanes of the group (lanes past the
     // last row evaluate pairs too), at most ceil(NPAIRS / G) each
     for (int k = lane; k < SODECL_NPAIRS; k += SODECL_G) {
@@ -640,6 +633,24 @@ SODECL_INLINE void sodecl_group_eval(double t, double* x, double* pp, int lane, 
 }
 #define SODECL_EVAL(T, X, F) sodecl_group_eval(T, X, pp, lane, F)
 #endif
+
+// Once per launch, before the first evaluation: the full pair matrix's
+// diagonal.  pair(i, i) = -pair(i, i) is zero for finite states (and a
+// non-finite state makes the whole row non-finite), so it is never evaluated,
+// and no evaluation writes the diagonal.
+SODECL_INLINE void sodecl_pairs_init(double* S, int lane)
+{
+#if SODECL_USE_PAIRS == 1
+#pragma unroll
+    for (int c = 0; c < SODECL_CPL; ++c) {
+        const int i = lane + c * SODECL_G;
+        if (i < _numeq_) S[i * SODECL_SSTRIDE + i] = 0.0;
+    }
+#else
+    (void)S;
+    (void)lane;
+#endif
+}
 
 // Writes this lane's components of v into LDS row x.
 SODECL_INLINE void sodecl_publish(double* x, const double (&v)[SODECL_CPL], int lane)
@@ -757,6 +768,7 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
     double* const S = nullptr;
 #endif
     (void)S;
+    sodecl_pairs_init(S, lane);
 
     for (int k = lane; k < _numpar_; k += SODECL_G)
         sh_p[slot][k] = active ? params[(size_t)orbit * _numpar_ + k] : 0.0;
