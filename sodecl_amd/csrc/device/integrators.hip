@@ -769,6 +769,10 @@ sodecl_group(double* __restrict__ t_state, double* __restrict__ y_state, const d
 #endif
     (void)S;
     sodecl_pairs_init(S, lane);
+#if SODECL_IS_SDE
+    (void)stage_ws;   // no adjoint of Euler-Maruyama: never stored
+    (void)ws_steps;
+#endif
 
     for (int k = lane; k < _numpar_; k += SODECL_G)
         sh_p[slot][k] = active ? params[(size_t)orbit * _numpar_ + k] : 0.0;
