@@ -29,7 +29,9 @@ static int check(size_t bytes, unsigned seed)
 int main()
 {
     const size_t mib = (size_t)1 << 20;
-    const size_t sizes[] = {0, 1, 63, 2 * mib - 1, 2 * mib, 4 * mib + 3, 16 * mib, 16 * mib + 7, 33 * mib + 5};
+    const size_t sl = sodecl_amd::kCopySliceMin;
+    const size_t sizes[] = {0, 1, 63, sl - 1, sl, sl + 3, 2 * sl - 1, 2 * sl, 4 * sl + 3, 16 * mib, 16 * mib + 7,
+                            33 * mib + 5};
     int bad = 0;
     for (size_t b : sizes) bad += check(b, 1);
     // several callers at once: jobs run one after another
