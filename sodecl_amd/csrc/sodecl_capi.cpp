@@ -225,18 +225,18 @@ void free_device(sodecl_ctx* c)
     hipFree(c->d_params);
     hipFree(c->d_counters);
     hipFree(c->d_newton);
-    hipFree(c->d_traj[0]);
-    hipFree(c->d_traj[1]);
-    hipHostFree(c->h_stage[0]);
-    hipHostFree(c->h_stage[1]);
+    for (int b = 0; b < sodecl_ctx::kTrajPipe; ++b) {
+        hipFree(c->d_traj[b]);
+        hipHostFree(c->h_stage[b]);
+        c->d_traj[b] = nullptr;
+        c->h_stage[b] = nullptr;
+    }
     if (c->ev_start) hipEventDestroy(c->ev_start);
     if (c->ev_stop) hipEventDestroy(c->ev_stop);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->copy_stream) hipStreamDestroy(c->copy_stream);
     c->d_t = c->d_y = c->d_params = c->d_counters = nullptr;
     c->d_newton = nullptr;
-    c->d_traj[0] = c->d_traj[1] = nullptr;
-    c->h_stage[0] = c->h_stage[1] = nullptr;
     c->ev_start = c->ev_stop = nullptr;
     c->stream = c->copy_stream = nullptr;
     c->traj_capacity = c->stage_capacity = 0;
@@ -387,10 +387,11 @@ int enqueue(sodecl_ctx* c, const LaunchCfg& cfg, double* t, double* y, const dou
 
 int ensure_traj(sodecl_ctx* c, size_t doubles, int nbuf)
 {
-    if (c->traj_capacity >= doubles && (nbuf < 2 || c->d_traj[1])) return 1;
-    hipFree(c->d_traj[0]);
-    hipFree(c->d_traj[1]);
-    c->d_traj[0] = c->d_traj[1] = nullptr;
+    if (c->traj_capacity >= doubles && c->d_traj[nbuf - 1]) return 1;
+    for (int b = 0; b < sodecl_ctx::kTrajPipe; ++b) {
+        hipFree(c->d_traj[b]);
+        c->d_traj[b] = nullptr;
+    }
     c->traj_capacity = 0;
     for (int b = 0; b < nbuf; ++b)
         if (!hip_ok(c, hipMalloc(&c->d_traj[b], doubles * sizeof(double)), "hipMalloc(trajectory)")) return 0;
@@ -398,14 +399,15 @@ int ensure_traj(sodecl_ctx* c, size_t doubles, int nbuf)
     return 1;
 }
 
-int ensure_stage(sodecl_ctx* c, size_t doubles)
+int ensure_stage(sodecl_ctx* c, size_t doubles, int nbuf)
 {
-    if (c->stage_capacity >= doubles) return 1;
-    hipHostFree(c->h_stage[0]);
-    hipHostFree(c->h_stage[1]);
-    c->h_stage[0] = c->h_stage[1] = nullptr;
+    if (c->stage_capacity >= doubles && c->h_stage[nbuf - 1]) return 1;
+    for (int b = 0; b < sodecl_ctx::kTrajPipe; ++b) {
+        hipHostFree(c->h_stage[b]);
+        c->h_stage[b] = nullptr;
+    }
     c->stage_capacity = 0;
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < nbuf; ++b)
         if (!hip_ok(c, hipHostMalloc((void**)&c->h_stage[b], doubles * sizeof(double), hipHostMallocDefault),
                     "hipHostMalloc"))
             return 0;
@@ -607,24 +609,27 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
         c->last_kernel_ms = ms;
         return 1;
     }
-    // segments per chunk within the buffer budget (two buffers when chunked)
+    // segments per chunk within the buffer budget (kTrajPipe buffers when
+    // chunked)
+    const int pipe = sodecl_ctx::kTrajPipe;
     long per = c->opts.segments_per_launch > 0 ? c->opts.segments_per_launch : launches;
     size_t budget_rows = std::max<size_t>(2, traj_budget(c) / (slab * sizeof(double)));
-    if ((size_t)per + 1 > budget_rows) per = (long)(budget_rows / 2) - 1;
+    if ((size_t)per + 1 > budget_rows) per = (long)(budget_rows / pipe) - 1;
     // A trajectory of more than 64 MiB for host memory streams in chunks of
-    // about 16 MiB: chunk k integrates while chunk k-1 crosses PCIe into
-    // pinned memory and chunk k-2 is copied into the caller's array by the
-    // host threads (one copy from device into pageable memory measured
-    // about 8.5 GB/s).  A host share needs the single launch.
+    // about 16 MiB: chunk k integrates while earlier chunks cross PCIe into
+    // pinned memory and are copied into the caller's array by the host
+    // threads (one copy from device into pageable memory measured about
+    // 8.5 GB/s).  Three buffers let each stage run a chunk ahead of the
+    // next one's jitter.  A host share needs the single launch.
     const size_t snap_bytes = slab * sizeof(double);
     if (host_orbits(c) == 0 && snap_bytes * (size_t)(launches + 1) > ((size_t)64 << 20))
         per = std::min<long>(per, std::max<long>(1, (long)(((size_t)16 << 20) / snap_bytes)));
     if (per < 1) per = 1;
     const bool single = per >= launches;
     const size_t rows = (size_t)std::min<long>(per, launches) + 1;
-    if (!ensure_traj(c, rows * slab, single ? 1 : 2)) return 0;
+    if (!ensure_traj(c, rows * slab, single ? 1 : pipe)) return 0;
     const bool to_array_direct = c->output_type == SODECL_OUTPUT_ARRAY && single;
-    if (!to_array_direct && !ensure_stage(c, rows * slab)) return 0;
+    if (!to_array_direct && !ensure_stage(c, rows * slab, single ? 1 : pipe)) return 0;
 
     if (single && host_orbits(c) > 0) return run_shared(c, launches, cfg, sink, to_array_direct);
     hipEventRecord(c->ev_start, c->stream);
@@ -644,26 +649,26 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
         else
             sink.put(host, nsnap * slab);
     } else {
-        // Chunk k runs on the compute stream into buffer k%2 while the copy
-        // stream moves chunk k-1 to pinned host memory and the host hands
-        // chunk k-2 to the sink.
-        hipEvent_t kdone[2] = {nullptr, nullptr}, cdone[2] = {nullptr, nullptr};
+        // Chunk k runs on the compute stream into buffer k % pipe while the
+        // copy stream moves the chunks before it to pinned host memory and
+        // the host hands chunk k - pipe to the sink.
+        hipEvent_t kdone[sodecl_ctx::kTrajPipe] = {}, cdone[sodecl_ctx::kTrajPipe] = {};
         bool ok = true;
-        for (int q = 0; ok && q < 2; ++q)
+        for (int q = 0; ok && q < pipe; ++q)
             ok = hip_ok(c, hipEventCreateWithFlags(&kdone[q], hipEventDisableTiming), "hipEventCreate") &&
                  hip_ok(c, hipEventCreateWithFlags(&cdone[q], hipEventDisableTiming), "hipEventCreate");
-        size_t rows_of[2] = {0, 0};
+        size_t rows_of[sodecl_ctx::kTrajPipe] = {};
         long k = 0;
         for (long s = 0; ok && s < launches; s += per, ++k) {
-            const int b = (int)(k & 1);
+            const int b = (int)(k % pipe);
             const int n = (int)std::min(per, launches - s);
             const bool last = s + n >= launches;
-            if (k >= 2) hipStreamWaitEvent(c->stream, cdone[b], 0);
+            if (k >= pipe) hipStreamWaitEvent(c->stream, cdone[b], 0);
             ok = enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, c->d_traj[b], c->d_newton, 0, n,
                          last ? 1 : 0, c->stream, nullptr, nullptr, 0, (long long)s * c->ksteps);
             if (!ok) break;
             hipEventRecord(kdone[b], c->stream);
-            if (k >= 2) {
+            if (k >= pipe) {
                 ok = hip_ok(c, hipEventSynchronize(cdone[b]), "copy");
                 if (!ok) break;
                 sink.put(c->h_stage[b], rows_of[b] * slab);
@@ -677,14 +682,14 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
             hipEventRecord(cdone[b], c->copy_stream);
         }
         hipEventRecord(c->ev_stop, c->stream);
-        // the last two chunks, oldest first
-        for (long q = std::max(0L, k - 2); ok && q < k; ++q) {
-            const int b = (int)(q & 1);
+        // the last chunks, oldest first
+        for (long q = std::max(0L, k - pipe); ok && q < k; ++q) {
+            const int b = (int)(q % pipe);
             ok = hip_ok(c, hipEventSynchronize(cdone[b]), "copy");
             if (ok) sink.put(c->h_stage[b], rows_of[b] * slab);
         }
         if (ok) ok = hip_ok(c, hipStreamSynchronize(c->stream), "kernel");
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < pipe; ++q) {
             if (kdone[q]) hipEventDestroy(kdone[q]);
             if (cdone[q]) hipEventDestroy(cdone[q]);
         }

@@ -137,9 +137,12 @@ struct sodecl_ctx {
     double* d_params = nullptr;
     double* d_counters = nullptr;
     long long* d_newton = nullptr;
-    double* d_traj[2] = {nullptr, nullptr};
+    // trajectory buffers: one for a single launch, kTrajPipe for the chunked
+    // copy-out (sodecl_capi.cpp: run_fast), with pinned staging alike
+    static constexpr int kTrajPipe = 3;
+    double* d_traj[kTrajPipe] = {};
     size_t traj_capacity = 0;       // doubles per trajectory buffer
-    double* h_stage[2] = {nullptr, nullptr};   // pinned staging for host output
+    double* h_stage[kTrajPipe] = {};   // pinned staging for host output
     size_t stage_capacity = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 
