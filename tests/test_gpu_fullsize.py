@@ -64,7 +64,7 @@ C2_STRICT_FLOOR = 0.9
 C2_STRICT_FLOOR_STRICT_ARITHMETIC = 0.95
 
 
-def c2_judge(gpu, ref, cfg, what):
+def c2_assess(gpu, ref, cfg, what):
     """Per-orbit verdicts of the conditioning-aware rule; returns the
     strict fraction of bounded orbits.  The spread of an orbit is how far
     five other valid roundings of the reference arithmetic move it from the
@@ -112,10 +112,10 @@ def test_c2_oculomotor_rk4_every_orbit():
                                    nthreads=host_threads())["out"]
     _, gpu = run_session(cfg)
     assert gpu.shape == (201, 6, 65536)
-    f_default = c2_judge(gpu, ref, cfg, "C2 default arithmetic")
+    f_default = c2_assess(gpu, ref, cfg, "C2 default arithmetic")
     del gpu
     _, gpu = run_session(cfg, fp_contract=1)
-    f_strict = c2_judge(gpu, ref, cfg, "C2 strict arithmetic (fp_contract=1)")
+    f_strict = c2_assess(gpu, ref, cfg, "C2 strict arithmetic (fp_contract=1)")
     assert f_default >= C2_STRICT_FLOOR, f_default
     assert f_strict >= C2_STRICT_FLOOR_STRICT_ARITHMETIC, f_strict
 
