@@ -193,10 +193,13 @@ def test_second_run_continues():
 def test_chunked_trajectory_matches_single_launch():
     cfg = small_oculomotor(2, orbits=500)
     _, one = run_session(cfg, schedule="lane")
-    # budget of a few snapshot rows forces many chunks and the two-buffer pipeline
-    budget = 3 * 6 * 500 * 8
-    _, many = run_session(cfg, schedule="lane", traj_budget_bytes=budget)
-    np.testing.assert_array_equal(one, many)
+    # budgets of a few snapshot rows force many chunks through the chunked
+    # pipeline (three buffers): one segment per chunk, then two and three
+    # (with a partial last chunk where the segment count does not divide)
+    for rows in (3, 9, 12):
+        budget = rows * 6 * 500 * 8
+        _, many = run_session(cfg, schedule="lane", traj_budget_bytes=budget)
+        np.testing.assert_array_equal(one, many, err_msg=f"budget of {rows} rows")
     _, segs = run_session(cfg, schedule="lane", segments_per_launch=7)
     np.testing.assert_array_equal(one, segs)
 
