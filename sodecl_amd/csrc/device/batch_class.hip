@@ -1,8 +1,8 @@
 // Per-size-class part of the batched implicit integrator.  The host emits
 // this text once per system size class, inside a namespace, after defining
-// _numeq_ and SODECL_LPO (lanes per orbit: 64, or 32 for classes of at most
-// 128 equations, two orbits per wavefront) for that class and including the
-// system plugin there.
+// _numeq_ and SODECL_LPO (lanes per orbit: 16 for classes of at most 32
+// equations, four orbits per wavefront; 32 up to 128 equations, two orbits;
+// else 64) for that class and including the system plugin there.
 //
 // One orbit is integrated by SODECL_LPO lanes of a wavefront:
 //   * components are owned strided (comp = lane + LPO c) for the right-hand

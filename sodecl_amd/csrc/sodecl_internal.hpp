@@ -82,19 +82,21 @@ bool compile_program(const std::string& text, const std::string& arch, int fp_co
                      std::string& log, bool ilp_schedule = false);
 std::string device_arch(int device);
 // Lanes of a wavefront per orbit in the batched integrator for a size class
-// of neq equations: 16 (four orbits per wavefront) up to 128 equations,
-// else 64.
-inline int batch_lanes_per_orbit(int neq) { return neq <= 128 ? 16 : 64; }
+// of neq equations: 16 (four orbits per wavefront) up to 32 equations, 32
+// (two orbits) up to 128, else 64.  The 128-equation class ran 2.5-3% faster
+// on 32 lanes (four rows per lane, 110 registers, four wavefronts per SIMD)
+// than on 16 (eight rows per lane, two wavefronts per SIMD by registers).
+inline int batch_lanes_per_orbit(int neq) { return neq <= 32 ? 16 : neq <= 128 ? 32 : 64; }
 // Occupancy hint of a size class's kernel (the wavefronts per SIMD its
-// register budget is sized for): up to two rows per lane need few registers
-// (four); a whole wavefront per orbit with up to 8 rows per lane fits 168
-// registers with no spill in its Newton loop (three); four orbits of 16
-// lanes with 8 rows per lane keep two (at three they also fit 168 registers,
-// but the launch of all classes side by side took 6-8% longer).
+// register budget is sized for): up to two rows per lane, or four rows on
+// 32 lanes, need few registers (four); a whole wavefront per orbit with up
+// to 8 rows per lane fits 168 registers with no spill in its Newton loop
+// (three); otherwise two.
 inline int batch_waves_per_simd(int neq)
 {
     const int lpo = batch_lanes_per_orbit(neq), rows = (neq + lpo - 1) / lpo;
     if (rows <= 2) return 4;
+    if (lpo == 32 && rows <= 4) return 4;
     return (lpo == 64 && rows <= 8) ? 3 : 2;
 }
 
