@@ -368,12 +368,14 @@ bool compile_program(const std::string& text, const std::string& arch, int fp_co
     // behind: the ILP scheduling strategy interleaves independent chains
     // (measured 3-4% on 65536-orbit oculomotor RK4, 2-3% on 1024-orbit
     // Kuramoto Euler).
-    if (ilp_schedule) {
+    // Extra compiler options, whitespace-separated (part of the cache key);
+    // a scheduling strategy among them replaces this one.
+    const char* extra = std::getenv("SODECL_AMD_JIT_OPTIONS");
+    if (ilp_schedule && !(extra && std::strstr(extra, "-amdgpu-sched-strategy="))) {
         opts.push_back("-mllvm");
         opts.push_back("-amdgpu-sched-strategy=max-ilp");
     }
-    // extra compiler options, whitespace-separated (part of the cache key)
-    if (const char* extra = std::getenv("SODECL_AMD_JIT_OPTIONS")) {
+    if (extra) {
         std::istringstream in(extra);
         for (std::string w; in >> w;) opts.push_back(w);
     }
