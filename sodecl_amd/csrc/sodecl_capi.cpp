@@ -663,23 +663,26 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
             const int b = (int)(k % pipe);
             const int n = (int)std::min(per, launches - s);
             const bool last = s + n >= launches;
-            if (k >= pipe) hipStreamWaitEvent(c->stream, cdone[b], 0);
+            if (k >= pipe && !hip_ok(c, hipStreamWaitEvent(c->stream, cdone[b], 0), "hipStreamWaitEvent")) {
+                ok = false;
+                break;
+            }
             ok = enqueue(c, cfg, c->d_t, c->d_y, c->d_params, c->d_counters, c->d_traj[b], c->d_newton, 0, n,
-                         last ? 1 : 0, c->stream, nullptr, nullptr, 0, (long long)s * c->ksteps);
+                         last ? 1 : 0, c->stream, nullptr, nullptr, 0, (long long)s * c->ksteps) &&
+                 hip_ok(c, hipEventRecord(kdone[b], c->stream), "hipEventRecord");
             if (!ok) break;
-            hipEventRecord(kdone[b], c->stream);
             if (k >= pipe) {
                 ok = hip_ok(c, hipEventSynchronize(cdone[b]), "copy");
                 if (!ok) break;
                 sink.put(c->h_stage[b], rows_of[b] * slab);
             }
-            hipStreamWaitEvent(c->copy_stream, kdone[b], 0);
             rows_of[b] = (size_t)n + (last ? 1 : 0);
-            ok = hip_ok(c, hipMemcpyAsync(c->h_stage[b], c->d_traj[b], rows_of[b] * slab * sizeof(double),
+            ok = hip_ok(c, hipStreamWaitEvent(c->copy_stream, kdone[b], 0), "hipStreamWaitEvent") &&
+                 hip_ok(c, hipMemcpyAsync(c->h_stage[b], c->d_traj[b], rows_of[b] * slab * sizeof(double),
                                           hipMemcpyDeviceToHost, c->copy_stream),
-                        "hipMemcpy(trajectory)");
+                        "hipMemcpy(trajectory)") &&
+                 hip_ok(c, hipEventRecord(cdone[b], c->copy_stream), "hipEventRecord");
             if (!ok) break;
-            hipEventRecord(cdone[b], c->copy_stream);
         }
         hipEventRecord(c->ev_stop, c->stream);
         // the last chunks, oldest first
@@ -689,6 +692,12 @@ int run_fast(sodecl_ctx* c, long launches, const LaunchCfg& cfg)
             if (ok) sink.put(c->h_stage[b], rows_of[b] * slab);
         }
         if (ok) ok = hip_ok(c, hipStreamSynchronize(c->stream), "kernel");
+        if (!ok) {
+            // the first error is kept; the copies already queued into the
+            // staging buffers must end before they can be reused
+            (void)hipStreamSynchronize(c->copy_stream);
+            (void)hipStreamSynchronize(c->stream);
+        }
         for (int q = 0; q < pipe; ++q) {
             if (kdone[q]) hipEventDestroy(kdone[q]);
             if (cdone[q]) hipEventDestroy(cdone[q]);
