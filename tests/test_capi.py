@@ -358,3 +358,32 @@ def test_host_share_library_matches_oracle(solver, name):
         np.testing.assert_array_equal(cnt[:first], counters[:first])
     if not explicit:
         np.testing.assert_array_equal(nu[first:], ref["newton_updates"][first:])
+
+
+def test_jit_options_reach_the_compiler_and_the_cache_key(tmp_path, monkeypatch):
+    """SODECL_AMD_JIT_OPTIONS is handed to hipRTC (an unknown option fails
+    the build) and is part of the code-object cache key (another option set
+    is compiled and stored beside the first); a scheduling strategy among
+    the options replaces the ILP strategy of one-wavefront launches instead
+    of being given a second time."""
+    monkeypatch.setenv("SODECL_AMD_CACHE_DIR", str(tmp_path))
+
+    def compile_with(opts):
+        if opts is None:
+            monkeypatch.delenv("SODECL_AMD_JIT_OPTIONS", raising=False)
+        else:
+            monkeypatch.setenv("SODECL_AMD_JIT_OPTIONS", opts)
+        s = si.Session(sodecl_amd.system_path("oculomotor"), 2, 65536, 6, 6, 0, 1e-3, 1.0, 100, 0,
+                       {"schedule": "lane"})
+        return s.compile_only("gfx950")
+
+    def stored():
+        return sorted(p.name for p in tmp_path.iterdir() if p.suffix == ".hsaco")
+
+    assert compile_with(None) > 1000
+    before = stored()
+    assert compile_with("-mllvm -amdgpu-sched-strategy=iterative-ilp") > 1000
+    after = stored()
+    assert len(after) == len(before) + 1
+    with pytest.raises(RuntimeError, match="unknown argument: '--sodecl-no-such-option'"):
+        compile_with("--sodecl-no-such-option")
